@@ -1,0 +1,804 @@
+// fft.hip -- batched row FFTs, the Fourier-space ramp filter and the staged
+// projection layout, for gfx950.
+//
+// Rows are transformed out of LDS: one workgroup owns one complex row of
+// length n <= 8192, loads it in bit-reversed order, runs log2(n) radix-2
+// butterfly stages against an LDS twiddle table and writes it back.  The
+// fused filter packs two real detector rows into one complex FFT (the gain
+// is real and even, so the real and imaginary parts of the inverse carry the
+// two filtered rows independently).
+#include <mutex>
+
+#include "ctr_internal.h"
+
+namespace ctr {
+
+constexpr int kMaxLog2 = 13;                 // 8192-point rows
+constexpr int kMaxN = 1 << kMaxLog2;
+constexpr int kFftThreads = 256;
+
+// exp(-2 pi i k / 8192), k < 4096, rounded from FP64.  Filled once per
+// device by init_twiddles; a row of length n uses every (8192/n)-th entry.
+__device__ float2 g_twiddle[kMaxN / 2];
+
+__global__ void init_twiddles_kernel()
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < kMaxN / 2) {
+        double s, c;
+        sincospi(-(double)k / (double)(kMaxN / 2), &s, &c);
+        g_twiddle[k] = make_float2((float)c, (float)s);
+    }
+}
+
+static std::mutex g_tw_lock;
+static bool g_tw_ready[64] = {false};
+
+static int ensure_twiddles()
+{
+    int dev = 0;
+    CTR_TRY_HIP(hipGetDevice(&dev));
+    CTR_REQUIRE(dev >= 0 && dev < 64, CTR_ERR_INVALID_ARGUMENT,
+                "device index %d out of range", dev);
+    std::lock_guard<std::mutex> g(g_tw_lock);
+    if (!g_tw_ready[dev]) {
+        init_twiddles_kernel<<<kMaxN / 2 / 256, 256, 0, 0>>>();
+        CTR_CHECK_LAUNCH();
+        CTR_TRY_HIP(hipDeviceSynchronize());
+        g_tw_ready[dev] = true;
+    }
+    return CTR_OK;
+}
+
+__device__ inline float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+template <int LOG2N>
+__device__ inline unsigned bitrev(unsigned e)
+{
+    return LOG2N == 0 ? 0u : (__brev(e) >> (32 - LOG2N));
+}
+
+// Loads the n/2 twiddles of an n-point transform into LDS.
+template <int LOG2N>
+__device__ inline void load_twiddles(float2 *T)
+{
+    constexpr int N = 1 << LOG2N;
+    constexpr int stride = kMaxN / N;
+    for (int k = threadIdx.x; k < N / 2; k += blockDim.x) T[k] = g_twiddle[k * stride];
+}
+
+// In-place radix-2 DIT stages over bit-reversed data in LDS.  INV selects
+// the conjugate twiddles (conj -> FFT -> conj of the reference equals this
+// exactly).  Ends with a barrier.
+template <int LOG2N, bool INV>
+__device__ inline void lds_stages(float2 *L, const float2 *T)
+{
+    constexpr int N = 1 << LOG2N;
+#pragma unroll 1
+    for (int s = 0; s < LOG2N; ++s) {
+        const int half = 1 << s;
+        const int tshift = LOG2N - 1 - s;  // twiddle index = pos << tshift
+        for (int b = threadIdx.x; b < N / 2; b += blockDim.x) {
+            int pos = b & (half - 1);
+            int i0 = ((b >> s) << (s + 1)) + pos;
+            int i1 = i0 + half;
+            float2 w = T[pos << tshift];
+            if (INV) w.y = -w.y;
+            float2 x0 = L[i0];
+            float2 t = cmul(L[i1], w);
+            L[i0] = make_float2(x0.x + t.x, x0.y + t.y);
+            L[i1] = make_float2(x0.x - t.x, x0.y - t.y);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Complex row FFT (FFFT::fft_radix2 / inverse_fft_radix2)
+// ---------------------------------------------------------------------------
+template <int LOG2N, bool INV>
+__global__ __launch_bounds__(kFftThreads) void fft_rows_kernel(float2 *data, int rows)
+{
+    constexpr int N = 1 << LOG2N;
+    __shared__ float2 L[N];
+    __shared__ float2 T[N / 2 > 0 ? N / 2 : 1];
+    const size_t base = (size_t)blockIdx.x * N;
+    load_twiddles<LOG2N>(T);
+    for (int e = threadIdx.x; e < N; e += blockDim.x) L[bitrev<LOG2N>(e)] = data[base + e];
+    __syncthreads();
+    lds_stages<LOG2N, INV>(L, T);
+    const float inv_n = 1.0f / (float)N;
+    for (int e = threadIdx.x; e < N; e += blockDim.x) {
+        float2 v = L[e];
+        if (INV) v = make_float2(v.x * inv_n, v.y * inv_n);
+        data[base + e] = v;
+    }
+}
+
+template <bool INV>
+static int launch_fft_rows(hipStream_t s, float2 *data, int log2n, int rows)
+{
+#define CTR_FFT_CASE(L)                                                          \
+    case L:                                                                      \
+        fft_rows_kernel<L, INV><<<rows, kFftThreads, 0, s>>>(data, rows);        \
+        break;
+    switch (log2n) {
+        CTR_FFT_CASE(0) CTR_FFT_CASE(1) CTR_FFT_CASE(2) CTR_FFT_CASE(3)
+        CTR_FFT_CASE(4) CTR_FFT_CASE(5) CTR_FFT_CASE(6) CTR_FFT_CASE(7)
+        CTR_FFT_CASE(8) CTR_FFT_CASE(9) CTR_FFT_CASE(10) CTR_FFT_CASE(11)
+        CTR_FFT_CASE(12) CTR_FFT_CASE(13)
+    default:
+        return fail(CTR_ERR_NOT_POWER_OF_TWO, "FFT length 2^%d unsupported", log2n);
+    }
+#undef CTR_FFT_CASE
+    CTR_CHECK_LAUNCH();
+    return CTR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Filter gain
+// ---------------------------------------------------------------------------
+// g(x) of FastFourierTransformImage.cpp:35-37 with true pi.
+__device__ inline float filter_g(int filter, double x)
+{
+    x = fabs(x);
+    if (filter == CTR_RAM_LAK) return (float)x;
+    if (filter == CTR_SHEPP_LOGAN) return (float)sin(x);
+    return (float)(fabs(sinpi(x)) / M_PI);
+}
+
+// Gain of the UNSHIFTED spectrum index k of an n-point row (n a power of
+// two): fft_shift moves k to i = (k + n/2) mod n, whose gain is
+// n * g((float)i/n - 0.5f) (FastFourierTransformImage.cpp:45), so the shift,
+// multiply and inverse shift of the reference collapse into one multiply.
+__device__ inline float unshifted_gain(int filter, int k, int n)
+{
+    int i = (k + n / 2) & (n - 1);
+    float x = (float)i / (float)n - 0.5f;
+    return (float)n * filter_g(filter, (double)x);
+}
+
+__global__ void apply_filter_kernel(float2 *spec, int n, long long total, int filter)
+{
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    int i = (int)(idx % n);
+    // FastFourierTransformImage.cpp:45: x = (float)i/n - 0.5f, times n
+    float x = (float)i / (float)n - 0.5f;
+    float g = (float)n * filter_g(filter, (double)x);
+    float2 v = spec[idx];
+    spec[idx] = make_float2(v.x * g, v.y * g);
+}
+
+// ---------------------------------------------------------------------------
+// Fused pad -> FFT -> gain -> IFFT -> crop over row pairs
+// ---------------------------------------------------------------------------
+struct FilterJob {
+    const float *in;
+    int width;          // samples per row (nu)
+    int rows;           // total rows
+    int filter;
+    // optional FDK pre-weight
+    int fdk;
+    int layout;         // CTR_LAYOUT_* of the input rows
+    int nv, n_views;
+    double sdd2, delta;
+    // output: plain rows (staged == nullptr) or fp32 staged layout
+    float *out;
+    float *staged;
+    int pitch_u, pitch_v;
+};
+
+__device__ inline void row_coords(const FilterJob &J, int r, int &view, int &iv)
+{
+    if (J.layout == CTR_LAYOUT_PROJECTIONS) {
+        iv = r % J.nv;
+        view = r / J.nv;
+    } else {
+        view = r % J.n_views;
+        iv = r / J.n_views;
+    }
+}
+
+template <int LOG2N>
+__global__ __launch_bounds__(kFftThreads) void filter_pairs_kernel(FilterJob J)
+{
+    constexpr int N = 1 << LOG2N;
+    __shared__ float2 L[N];
+    __shared__ float2 T[N / 2 > 0 ? N / 2 : 1];
+    constexpr int PER = (N + kFftThreads - 1) / kFftThreads;
+
+    const int r0 = 2 * blockIdx.x, r1 = r0 + 1;
+    const bool has1 = r1 < J.rows;
+    int view0 = 0, iv0 = 0, view1 = 0, iv1 = 0;
+    row_coords(J, r0, view0, iv0);
+    if (has1) row_coords(J, r1, view1, iv1);
+    double vv0 = 0.0, vv1 = 0.0, cu = 0.5 * J.width - 0.5;
+    if (J.fdk) {
+        double v0 = (iv0 + 0.5 - 0.5 * J.nv) * J.delta;
+        double v1 = (iv1 + 0.5 - 0.5 * J.nv) * J.delta;
+        vv0 = J.sdd2 + v0 * v0;
+        vv1 = J.sdd2 + v1 * v1;
+    }
+    const float *a_row = J.in + (size_t)r0 * J.width;
+    const float *b_row = J.in + (size_t)r1 * J.width;
+
+    load_twiddles<LOG2N>(T);
+    for (int e = threadIdx.x; e < N; e += blockDim.x) {
+        float a = 0.f, b = 0.f;
+        if (e < J.width) {
+            a = a_row[e];
+            if (has1) b = b_row[e];
+            if (J.fdk) {
+                double u = (e - cu) * J.delta;
+                double sdd = sqrt(J.sdd2);
+                a = (float)((double)a * (sdd / sqrt(vv0 + u * u)));
+                b = (float)((double)b * (sdd / sqrt(vv1 + u * u)));
+            }
+        }
+        L[bitrev<LOG2N>(e)] = make_float2(a, b);
+    }
+    __syncthreads();
+    lds_stages<LOG2N, false>(L, T);
+
+    // gain, then bit-reverse again for the inverse transform
+    float2 reg[PER];
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        int k = threadIdx.x + p * kFftThreads;
+        if (k < N) {
+            float g = unshifted_gain(J.filter, k, N);
+            float2 v = L[k];
+            reg[p] = make_float2(v.x * g, v.y * g);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        int k = threadIdx.x + p * kFftThreads;
+        if (k < N) L[bitrev<LOG2N>(k)] = reg[p];
+    }
+    __syncthreads();
+    lds_stages<LOG2N, true>(L, T);
+
+    const float inv_n = 1.0f / (float)N;
+    if (J.staged == nullptr) {
+        float *oa = J.out + (size_t)r0 * J.width;
+        float *ob = J.out + (size_t)r1 * J.width;
+        for (int e = threadIdx.x; e < J.width; e += blockDim.x) {
+            float2 v = L[e];
+            oa[e] = v.x * inv_n;
+            if (has1) ob[e] = v.y * inv_n;
+        }
+    } else {
+        size_t vstride = (size_t)J.pitch_u * J.pitch_v;
+        float *oa = J.staged + (size_t)view0 * vstride + (size_t)(iv0 + 1) * J.pitch_u + 1;
+        float *ob = J.staged + (size_t)view1 * vstride + (size_t)(iv1 + 1) * J.pitch_u + 1;
+        for (int e = threadIdx.x; e < J.width; e += blockDim.x) {
+            float2 v = L[e];
+            oa[e] = v.x * inv_n;
+            if (has1) ob[e] = v.y * inv_n;
+        }
+    }
+}
+
+static int launch_filter_pairs(hipStream_t s, const FilterJob &J, int log2n)
+{
+    int blocks = (J.rows + 1) / 2;
+    if (blocks == 0) return CTR_OK;
+#define CTR_FILT_CASE(L)                                                          \
+    case L:                                                                       \
+        filter_pairs_kernel<L><<<blocks, kFftThreads, 0, s>>>(J);                 \
+        break;
+    switch (log2n) {
+        CTR_FILT_CASE(0) CTR_FILT_CASE(1) CTR_FILT_CASE(2) CTR_FILT_CASE(3)
+        CTR_FILT_CASE(4) CTR_FILT_CASE(5) CTR_FILT_CASE(6) CTR_FILT_CASE(7)
+        CTR_FILT_CASE(8) CTR_FILT_CASE(9) CTR_FILT_CASE(10) CTR_FILT_CASE(11)
+        CTR_FILT_CASE(12) CTR_FILT_CASE(13)
+    default:
+        return fail(CTR_ERR_NOT_POWER_OF_TWO, "FFT length 2^%d unsupported", log2n);
+    }
+#undef CTR_FILT_CASE
+    CTR_CHECK_LAUNCH();
+    return CTR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Staged layout helpers
+// ---------------------------------------------------------------------------
+// Zeroes the border texels of an fp32 staged buffer (rows 0, nv+1, nv+2 and
+// columns 0, nu+1 .. pitch_u-1 of every view).
+__global__ void zero_border_kernel(float *st, int nu, int nv, int n_views,
+                                   int pitch_u, int pitch_v)
+{
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long total = (long long)pitch_u * pitch_v * n_views;
+    if (idx >= total) return;
+    int c = (int)(idx % pitch_u);
+    int r = (int)((idx / pitch_u) % pitch_v);
+    bool interior = r >= 1 && r <= nv && c >= 1 && c <= nu;
+    if (!interior) st[idx] = 0.0f;
+}
+
+// Copies plain projections (either layout) into an fp32 staged buffer,
+// borders included.
+__global__ void stage_plain_kernel(const float *proj, int layout, int nu, int nv,
+                                   int n_views, float *st, int pitch_u, int pitch_v)
+{
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long total = (long long)pitch_u * pitch_v * n_views;
+    if (idx >= total) return;
+    int c = (int)(idx % pitch_u);
+    int r = (int)((idx / pitch_u) % pitch_v);
+    int n = (int)(idx / ((long long)pitch_u * pitch_v));
+    float v = 0.0f;
+    int iu = c - 1, iv = r - 1;
+    if (iu >= 0 && iu < nu && iv >= 0 && iv < nv) {
+        size_t src = layout == CTR_LAYOUT_PROJECTIONS
+                         ? ((size_t)n * nv + iv) * nu + iu
+                         : ((size_t)iv * n_views + n) * nu + iu;
+        v = proj[src];
+    }
+    st[idx] = v;
+}
+
+// Per-view max |value| of an fp32 staged buffer: one block per view.
+__global__ __launch_bounds__(256) void view_max_kernel(const float *st, size_t view_texels,
+                                                       float *view_scale)
+{
+    const float *p = st + (size_t)blockIdx.x * view_texels;
+    float m = 0.0f;
+    for (size_t t = threadIdx.x; t < view_texels; t += blockDim.x) m = fmaxf(m, fabsf(p[t]));
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    __shared__ float part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
+        view_scale[blockIdx.x] = m > 0.0f ? m : 1.0f;
+    }
+}
+
+__device__ inline unsigned short f32_to_bf16_rne(float f)
+{
+    unsigned u = __float_as_uint(f);
+    unsigned lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;
+    return (unsigned short)(u >> 16);
+}
+
+__device__ inline unsigned short f32_to_f16_bits(float f)
+{
+    _Float16 h = (_Float16)f;
+    return __builtin_bit_cast(unsigned short, h);
+}
+
+// fp32 staged -> 16-bit pair texels: texel c of a row holds
+// (q[c], q[c+1]) / view_scale, so one aligned 32-bit load gives both
+// horizontal bilinear taps.
+__global__ void quantize_pairs_kernel(const float *st, int pitch_u, int pitch_v,
+                                      int n_views, const float *view_scale,
+                                      unsigned *out, int bf16)
+{
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long total = (long long)pitch_u * pitch_v * n_views;
+    if (idx >= total) return;
+    int c = (int)(idx % pitch_u);
+    int n = (int)(idx / ((long long)pitch_u * pitch_v));
+    float inv = 1.0f / view_scale[n];
+    float a = st[idx] * inv;
+    float b = c + 1 < pitch_u ? st[idx + 1] * inv : 0.0f;
+    unsigned lo, hi;
+    if (bf16) {
+        lo = f32_to_bf16_rne(a);
+        hi = f32_to_bf16_rne(b);
+    } else {
+        lo = f32_to_f16_bits(a);
+        hi = f32_to_f16_bits(b);
+    }
+    out[idx] = lo | (hi << 16);
+}
+
+static inline unsigned grid1d(long long total, int block)
+{
+    return (unsigned)((total + block - 1) / block);
+}
+
+// fp32 staged (borders zero) -> caller's staged buffer in `dtype`.
+static int finish_staged(hipStream_t s, const float *st32, int nu, int nv, int n_views,
+                         void *staged, int dtype, float *view_scale)
+{
+    StagedLayout L = staged_layout(nu, nv, n_views, dtype);
+    if (dtype == CTR_F32) return CTR_OK;
+    CTR_REQUIRE(view_scale, CTR_ERR_NULL_POINTER,
+                "view_scale is required for 16-bit staged projections");
+    view_max_kernel<<<n_views, 256, 0, s>>>(st32, L.view, view_scale);
+    CTR_CHECK_LAUNCH();
+    long long total = (long long)L.view * n_views;
+    quantize_pairs_kernel<<<grid1d(total, 256), 256, 0, s>>>(
+        st32, L.pitch_u, L.pitch_v, n_views, view_scale, (unsigned *)staged,
+        dtype == CTR_BF16);
+    CTR_CHECK_LAUNCH();
+    return CTR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise complex helpers
+// ---------------------------------------------------------------------------
+__global__ void shift_rows_kernel(const float2 *src, float2 *dst, int n, int rows, int shift)
+{
+    // one block row-chunk; out-of-place semantics via LDS-free gather:
+    // dst[i] = src[(i - shift) mod n]
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long total = (long long)n * rows;
+    if (idx >= total) return;
+    int i = (int)(idx % n);
+    long long row = idx / n;
+    int sh = shift % n;
+    if (sh < 0) sh += n;
+    int j = i - sh;
+    if (j < 0) j += n;
+    dst[idx] = src[row * n + j];
+}
+
+__global__ void blit_kernel(int mode, const float *src, int sw, float *dst, int dw,
+                            int w, int h)
+{
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)w * h) return;
+    int x = (int)(idx % w), y = (int)(idx / w);
+    size_t si = (size_t)y * sw + x, di = (size_t)y * dw + x;
+    switch (mode) {
+    case CTR_BLIT_REAL_TO_COMPLEX:
+        dst[2 * di] = src[si];
+        dst[2 * di + 1] = 0.0f;
+        break;
+    case CTR_BLIT_COMPLEX_TO_COMPLEX:
+        dst[2 * di] = src[2 * si];
+        dst[2 * di + 1] = src[2 * si + 1];
+        break;
+    case CTR_BLIT_COMPLEX_TO_REAL_R:
+        dst[di] = src[2 * si];
+        break;
+    default:
+        dst[di] = src[2 * si + 1];
+        break;
+    }
+}
+
+__global__ void conj_scale_kernel(float2 *data, long long count, float conj_sign, float mul)
+{
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= count) return;
+    float2 v = data[idx];
+    data[idx] = make_float2(v.x * mul, v.y * conj_sign * mul);
+}
+
+__global__ void bit_reverse_kernel(float2 *data, int n, int log2n, long long total)
+{
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    int i = (int)(idx % n);
+    unsigned j = log2n == 0 ? 0u : (__brev((unsigned)i) >> (32 - log2n));
+    if ((int)j > i) {
+        size_t row = (size_t)(idx / n) * n;
+        float2 t = data[row + i];
+        data[row + i] = data[row + j];
+        data[row + j] = t;
+    }
+}
+
+__global__ void single_stage_kernel(const float2 *rd, float2 *wr, int n, long long total,
+                                    int size)
+{
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    int i = (int)(idx % n);
+    size_t row = (size_t)(idx / n) * n;
+    int half = size / 2, local = i % size;
+    int k = (local % half) * (n / size);
+    float2 w = g_twiddle[k * (kMaxN / n)];
+    if (local >= half) {
+        float2 a = rd[row + i - half], t = cmul(rd[row + i], w);
+        wr[row + i] = make_float2(a.x - t.x, a.y - t.y);
+    } else {
+        float2 a = rd[row + i], t = cmul(rd[row + i + half], w);
+        wr[row + i] = make_float2(a.x + t.x, a.y + t.y);
+    }
+}
+
+__global__ void scale_real_kernel(float *data, long long count, float mul)
+{
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < count) data[idx] *= mul;
+}
+
+}  // namespace ctr
+
+using namespace ctr;
+
+extern "C" {
+
+int ctr_fft_radix2(void *stream, float *data, int n, int rows)
+{
+    clear_error();
+    CTR_REQUIRE(data, CTR_ERR_NULL_POINTER, "ctr_fft_radix2: data is NULL");
+    CTR_REQUIRE(is_pow2(n), CTR_ERR_NOT_POWER_OF_TWO,
+                "[CTAnalyzer Error] FFFT::fft_radix2() is called but texture's width "
+                "(%d) is not a power of 2", n);
+    CTR_REQUIRE(n <= kMaxN && rows >= 0, CTR_ERR_INVALID_ARGUMENT,
+                "ctr_fft_radix2: n=%d rows=%d outside [1, %d] x [0, inf)", n, rows, kMaxN);
+    if (rows == 0) return CTR_OK;
+    int rc = ensure_twiddles();
+    if (rc) return rc;
+    return launch_fft_rows<false>(as_stream(stream), (float2 *)data, log2_exact(n), rows);
+}
+
+int ctr_inverse_fft_radix2(void *stream, float *data, int n, int rows)
+{
+    clear_error();
+    CTR_REQUIRE(data, CTR_ERR_NULL_POINTER, "ctr_inverse_fft_radix2: data is NULL");
+    CTR_REQUIRE(is_pow2(n), CTR_ERR_NOT_POWER_OF_TWO,
+                "[CTAnalyzer Error] FFFT::fft_radix2() is called but texture's width "
+                "(%d) is not a power of 2", n);
+    CTR_REQUIRE(n <= kMaxN && rows >= 0, CTR_ERR_INVALID_ARGUMENT,
+                "ctr_inverse_fft_radix2: n=%d rows=%d out of range", n, rows);
+    if (rows == 0) return CTR_OK;
+    int rc = ensure_twiddles();
+    if (rc) return rc;
+    return launch_fft_rows<true>(as_stream(stream), (float2 *)data, log2_exact(n), rows);
+}
+
+int ctr_fft_shift(void *stream, const float *src, float *dst, int n, int rows, int shift)
+{
+    clear_error();
+    CTR_REQUIRE(src && dst, CTR_ERR_NULL_POINTER, "ctr_fft_shift: NULL buffer");
+    CTR_REQUIRE(n > 0 && rows >= 0, CTR_ERR_INVALID_ARGUMENT, "ctr_fft_shift: bad size");
+    long long total = (long long)n * rows;
+    if (total == 0) return CTR_OK;
+    hipStream_t s = as_stream(stream);
+    if (src == dst) {
+        // in-place overload (FastFourierTransformGPU.cpp:113-119): shift into
+        // a temporary, then copy back
+        float2 *tmp = nullptr;
+        CTR_TRY_HIP(hipMallocAsync((void **)&tmp, sizeof(float2) * total, s));
+        shift_rows_kernel<<<grid1d(total, 256), 256, 0, s>>>((const float2 *)src, tmp, n, rows, shift);
+        CTR_CHECK_LAUNCH();
+        CTR_TRY_HIP(hipMemcpyAsync(dst, tmp, sizeof(float2) * total, hipMemcpyDeviceToDevice, s));
+        CTR_TRY_HIP(hipFreeAsync(tmp, s));
+        return CTR_OK;
+    }
+    shift_rows_kernel<<<grid1d(total, 256), 256, 0, s>>>((const float2 *)src, (float2 *)dst, n,
+                                                         rows, shift);
+    CTR_CHECK_LAUNCH();
+    return CTR_OK;
+}
+
+int ctr_blit(void *stream, int mode, const float *src, int src_width, int src_height,
+             float *dst, int dst_width, int dst_height)
+{
+    clear_error();
+    CTR_REQUIRE(src && dst, CTR_ERR_NULL_POINTER, "ctr_blit: NULL buffer");
+    CTR_REQUIRE(mode >= 0 && mode <= 3, CTR_ERR_INVALID_ARGUMENT, "ctr_blit: bad mode %d", mode);
+    CTR_REQUIRE(src_width >= 0 && src_height >= 0 && dst_width >= 0 && dst_height >= 0,
+                CTR_ERR_INVALID_ARGUMENT, "ctr_blit: negative extent");
+    int w = src_width < dst_width ? src_width : dst_width;
+    int h = src_height < dst_height ? src_height : dst_height;
+    long long total = (long long)w * h;
+    if (total == 0) return CTR_OK;
+    blit_kernel<<<grid1d(total, 256), 256, 0, as_stream(stream)>>>(mode, src, src_width, dst,
+                                                                   dst_width, w, h);
+    CTR_CHECK_LAUNCH();
+    return CTR_OK;
+}
+
+int ctr_conjugate(void *stream, float *data, size_t count)
+{
+    clear_error();
+    CTR_REQUIRE(data || count == 0, CTR_ERR_NULL_POINTER, "ctr_conjugate: NULL buffer");
+    if (count == 0) return CTR_OK;
+    conj_scale_kernel<<<grid1d((long long)count, 256), 256, 0, as_stream(stream)>>>(
+        (float2 *)data, (long long)count, -1.0f, 1.0f);
+    CTR_CHECK_LAUNCH();
+    return CTR_OK;
+}
+
+int ctr_scale_complex(void *stream, float *data, size_t count, float multiplier)
+{
+    clear_error();
+    CTR_REQUIRE(data || count == 0, CTR_ERR_NULL_POINTER, "ctr_scale_complex: NULL buffer");
+    if (count == 0) return CTR_OK;
+    conj_scale_kernel<<<grid1d((long long)count, 256), 256, 0, as_stream(stream)>>>(
+        (float2 *)data, (long long)count, 1.0f, multiplier);
+    CTR_CHECK_LAUNCH();
+    return CTR_OK;
+}
+
+int ctr_fft_bit_reverse(void *stream, float *data, int n, int rows)
+{
+    clear_error();
+    CTR_REQUIRE(data, CTR_ERR_NULL_POINTER, "ctr_fft_bit_reverse: data is NULL");
+    CTR_REQUIRE(is_pow2(n), CTR_ERR_NOT_POWER_OF_TWO, "ctr_fft_bit_reverse: %d is not a power of 2", n);
+    CTR_REQUIRE(rows >= 0, CTR_ERR_INVALID_ARGUMENT, "ctr_fft_bit_reverse: rows < 0");
+    long long total = (long long)n * rows;
+    if (total == 0) return CTR_OK;
+    bit_reverse_kernel<<<grid1d(total, 256), 256, 0, as_stream(stream)>>>((float2 *)data, n,
+                                                                          log2_exact(n), total);
+    CTR_CHECK_LAUNCH();
+    return CTR_OK;
+}
+
+int ctr_fft_single_stage(void *stream, const float *read, float *write, int n, int rows,
+                         int step_index)
+{
+    clear_error();
+    CTR_REQUIRE(read && write && read != write, CTR_ERR_INVALID_ARGUMENT,
+                "ctr_fft_single_stage: read and write must be distinct, non-NULL buffers");
+    CTR_REQUIRE(is_pow2(n) && n <= kMaxN, CTR_ERR_NOT_POWER_OF_TWO,
+                "ctr_fft_single_stage: n=%d must be a power of 2 <= %d", n, kMaxN);
+    CTR_REQUIRE(rows >= 0 && step_index >= 0 && (2 << step_index) <= n, CTR_ERR_INVALID_ARGUMENT,
+                "ctr_fft_single_stage: step %d out of range for n=%d", step_index, n);
+    long long total = (long long)n * rows;
+    if (total == 0) return CTR_OK;
+    int rc = ensure_twiddles();
+    if (rc) return rc;
+    single_stage_kernel<<<grid1d(total, 256), 256, 0, as_stream(stream)>>>(
+        (const float2 *)read, (float2 *)write, n, total, 2 << step_index);
+    CTR_CHECK_LAUNCH();
+    return CTR_OK;
+}
+
+int ctr_scale_real(void *stream, float *data, size_t count, float multiplier)
+{
+    clear_error();
+    CTR_REQUIRE(data || count == 0, CTR_ERR_NULL_POINTER, "ctr_scale_real: NULL buffer");
+    if (count == 0) return CTR_OK;
+    scale_real_kernel<<<grid1d((long long)count, 256), 256, 0, as_stream(stream)>>>(
+        data, (long long)count, multiplier);
+    CTR_CHECK_LAUNCH();
+    return CTR_OK;
+}
+
+int ctr_apply_filter(void *stream, float *spectrum, int n, int rows, int filter)
+{
+    clear_error();
+    CTR_REQUIRE(spectrum, CTR_ERR_NULL_POINTER, "ctr_apply_filter: NULL buffer");
+    CTR_REQUIRE(filter >= CTR_RAM_LAK && filter <= CTR_COSINE, CTR_ERR_INVALID_ARGUMENT,
+                "[CTAnalyzer Error] FBP2D::apply_filter() unknown filter type %d", filter);
+    CTR_REQUIRE(n > 0 && rows >= 0, CTR_ERR_INVALID_ARGUMENT, "ctr_apply_filter: bad size");
+    long long total = (long long)n * rows;
+    if (total == 0) return CTR_OK;
+    apply_filter_kernel<<<grid1d(total, 256), 256, 0, as_stream(stream)>>>(
+        (float2 *)spectrum, n, total, filter);
+    CTR_CHECK_LAUNCH();
+    return CTR_OK;
+}
+
+int ctr_filter_rows(void *stream, const float *in, float *out, int width, int rows,
+                    int fft_length, int filter)
+{
+    clear_error();
+    CTR_REQUIRE(in && out, CTR_ERR_NULL_POINTER, "ctr_filter_rows: NULL buffer");
+    CTR_REQUIRE(width > 0 && rows >= 0, CTR_ERR_INVALID_ARGUMENT,
+                "ctr_filter_rows: width=%d rows=%d", width, rows);
+    CTR_REQUIRE(filter >= CTR_RAM_LAK && filter <= CTR_COSINE, CTR_ERR_INVALID_ARGUMENT,
+                "[CTAnalyzer Error] FBP2D::apply_filter() unknown filter type %d", filter);
+    int n = fft_length == 0 ? ceil_pow2(width) : fft_length;
+    CTR_REQUIRE(is_pow2(n), CTR_ERR_NOT_POWER_OF_TWO,
+                "ctr_filter_rows: FFT length %d is not a power of 2", n);
+    CTR_REQUIRE(n >= width && n <= kMaxN, CTR_ERR_INVALID_ARGUMENT,
+                "ctr_filter_rows: FFT length %d must lie in [width=%d, %d]", n, width, kMaxN);
+    if (rows == 0) return CTR_OK;
+    int rc = ensure_twiddles();
+    if (rc) return rc;
+    FilterJob J{};
+    J.in = in;
+    J.width = width;
+    J.rows = rows;
+    J.filter = filter;
+    J.fdk = 0;
+    J.layout = CTR_LAYOUT_PROJECTIONS;
+    J.nv = 1;
+    J.n_views = 1;
+    J.out = out;
+    J.staged = nullptr;
+    return launch_filter_pairs(as_stream(stream), J, log2_exact(n));
+}
+
+int ctr_filter_projections(void *stream, ctr_workspace *ws, const float *proj, int layout,
+                           int nu, int nv, int n_views, int fft_length, int filter,
+                           int fdk_weight, double source_distance, double detector_distance,
+                           double detector_width, void *staged, int staged_dtype,
+                           float *view_scale)
+{
+    clear_error();
+    CTR_REQUIRE(proj && staged, CTR_ERR_NULL_POINTER, "ctr_filter_projections: NULL buffer");
+    CTR_REQUIRE(nu > 0 && nv > 0 && n_views > 0, CTR_ERR_INVALID_ARGUMENT,
+                "ctr_filter_projections: sizes (%d, %d, %d)", nu, nv, n_views);
+    CTR_REQUIRE(layout == CTR_LAYOUT_PROJECTIONS || layout == CTR_LAYOUT_SINOGRAMS,
+                CTR_ERR_INVALID_ARGUMENT, "unknown layout %d", layout);
+    CTR_REQUIRE(staged_dtype == CTR_F32 || staged_dtype == CTR_F16 || staged_dtype == CTR_BF16,
+                CTR_ERR_INVALID_ARGUMENT, "unknown staged dtype %d", staged_dtype);
+    CTR_REQUIRE(filter >= CTR_RAM_LAK && filter <= CTR_COSINE, CTR_ERR_INVALID_ARGUMENT,
+                "[CTAnalyzer Error] FBP2D::apply_filter() unknown filter type %d", filter);
+    if (fdk_weight)
+        CTR_REQUIRE(source_distance > 0 && detector_distance >= 0 && detector_width > 0,
+                    CTR_ERR_INVALID_ARGUMENT, "ctr_filter_projections: bad FDK geometry");
+    int n = fft_length == 0 ? ceil_pow2(nu) : fft_length;
+    CTR_REQUIRE(is_pow2(n), CTR_ERR_NOT_POWER_OF_TWO, "FFT length %d is not a power of 2", n);
+    CTR_REQUIRE(n >= nu && n <= kMaxN, CTR_ERR_INVALID_ARGUMENT,
+                "FFT length %d must lie in [nu=%d, %d]", n, nu, kMaxN);
+    ctr_workspace *w = nullptr;
+    int rc = workspace_resolve(ws, &w);
+    if (rc) return rc;
+    if ((rc = ensure_twiddles())) return rc;
+    hipStream_t s = as_stream(stream);
+    StagedLayout L = staged_layout(nu, nv, n_views, CTR_F32);
+    float *st32 = nullptr;
+    if (staged_dtype == CTR_F32) {
+        st32 = (float *)staged;
+    } else {
+        void *p = nullptr;
+        if ((rc = workspace_get(w, SLOT_STAGE_F32, L.bytes, &p))) return rc;
+        st32 = (float *)p;
+    }
+    long long total = (long long)L.view * n_views;
+    zero_border_kernel<<<grid1d(total, 256), 256, 0, s>>>(st32, nu, nv, n_views, L.pitch_u,
+                                                          L.pitch_v);
+    CTR_CHECK_LAUNCH();
+    FilterJob J{};
+    J.in = proj;
+    J.width = nu;
+    J.rows = nv * n_views;
+    J.filter = filter;
+    J.fdk = fdk_weight ? 1 : 0;
+    J.layout = layout;
+    J.nv = nv;
+    J.n_views = n_views;
+    double sdd = source_distance + detector_distance;
+    J.sdd2 = sdd * sdd;
+    J.delta = detector_width / nu;
+    J.out = nullptr;
+    J.staged = st32;
+    J.pitch_u = L.pitch_u;
+    J.pitch_v = L.pitch_v;
+    if ((rc = launch_filter_pairs(s, J, log2_exact(n)))) return rc;
+    return finish_staged(s, st32, nu, nv, n_views, staged, staged_dtype, view_scale);
+}
+
+int ctr_stage_projections(void *stream, ctr_workspace *ws, const float *proj, int layout,
+                          int nu, int nv, int n_views, void *staged, int staged_dtype,
+                          float *view_scale)
+{
+    clear_error();
+    CTR_REQUIRE(proj && staged, CTR_ERR_NULL_POINTER, "ctr_stage_projections: NULL buffer");
+    CTR_REQUIRE(nu > 0 && nv > 0 && n_views > 0, CTR_ERR_INVALID_ARGUMENT,
+                "ctr_stage_projections: sizes (%d, %d, %d)", nu, nv, n_views);
+    CTR_REQUIRE(layout == CTR_LAYOUT_PROJECTIONS || layout == CTR_LAYOUT_SINOGRAMS,
+                CTR_ERR_INVALID_ARGUMENT, "unknown layout %d", layout);
+    CTR_REQUIRE(staged_dtype == CTR_F32 || staged_dtype == CTR_F16 || staged_dtype == CTR_BF16,
+                CTR_ERR_INVALID_ARGUMENT, "unknown staged dtype %d", staged_dtype);
+    ctr_workspace *w = nullptr;
+    int rc = workspace_resolve(ws, &w);
+    if (rc) return rc;
+    hipStream_t s = as_stream(stream);
+    StagedLayout L = staged_layout(nu, nv, n_views, CTR_F32);
+    float *st32 = nullptr;
+    if (staged_dtype == CTR_F32) {
+        st32 = (float *)staged;
+    } else {
+        void *p = nullptr;
+        if ((rc = workspace_get(w, SLOT_STAGE_F32, L.bytes, &p))) return rc;
+        st32 = (float *)p;
+    }
+    long long total = (long long)L.view * n_views;
+    stage_plain_kernel<<<grid1d(total, 256), 256, 0, s>>>(proj, layout, nu, nv, n_views, st32,
+                                                          L.pitch_u, L.pitch_v);
+    CTR_CHECK_LAUNCH();
+    return finish_staged(s, st32, nu, nv, n_views, staged, staged_dtype, view_scale);
+}
+
+}  // extern "C"

@@ -1,0 +1,501 @@
+// forward.hip -- ray-driven forward projection, fan->parallel rebinning and
+// the elementwise detector / volume passes, for gfx950.
+#include <cmath>
+#include <vector>
+
+#include "ctr_internal.h"
+
+namespace ctr {
+
+static inline unsigned grid1d(long long total, int block)
+{
+    return (unsigned)((total + block - 1) / block);
+}
+
+// ---------------------------------------------------------------------------
+// Forward projection
+// ---------------------------------------------------------------------------
+__device__ inline float vol_at(const float *__restrict__ vol, int X, int Y, int Z, int i, int j,
+                               int k)
+{
+    return (i >= 0 && i < X && j >= 0 && j < Y && k >= 0 && k < Z)
+               ? vol[((size_t)k * Y + j) * X + i]
+               : 0.0f;
+}
+
+// Trilinear sample at continuous voxel-index coordinates, zero outside.
+__device__ inline float trilinear0(const float *__restrict__ vol, int X, int Y, int Z, float fi,
+                                   float fj, float fk)
+{
+    if (!(fi > -1.0f && fi < (float)X && fj > -1.0f && fj < (float)Y && fk > -1.0f &&
+          fk < (float)Z))
+        return 0.0f;
+    const float i0f = floorf(fi), j0f = floorf(fj), k0f = floorf(fk);
+    const int i0 = (int)i0f, j0 = (int)j0f, k0 = (int)k0f;
+    const float wi = fi - i0f, wj = fj - j0f, wk = fk - k0f;
+    float c00, c01, c10, c11;
+    if (i0 >= 0 && i0 + 1 < X && j0 >= 0 && j0 + 1 < Y && k0 >= 0 && k0 + 1 < Z) {
+        const float *p = vol + ((size_t)k0 * Y + j0) * X + i0;
+        const size_t sy = X, sz = (size_t)X * Y;
+        c00 = fmaf(wi, p[1] - p[0], p[0]);
+        c01 = fmaf(wi, p[sy + 1] - p[sy], p[sy]);
+        c10 = fmaf(wi, p[sz + 1] - p[sz], p[sz]);
+        c11 = fmaf(wi, p[sz + sy + 1] - p[sz + sy], p[sz + sy]);
+    } else {
+        float a, b;
+        a = vol_at(vol, X, Y, Z, i0, j0, k0);
+        b = vol_at(vol, X, Y, Z, i0 + 1, j0, k0);
+        c00 = fmaf(wi, b - a, a);
+        a = vol_at(vol, X, Y, Z, i0, j0 + 1, k0);
+        b = vol_at(vol, X, Y, Z, i0 + 1, j0 + 1, k0);
+        c01 = fmaf(wi, b - a, a);
+        a = vol_at(vol, X, Y, Z, i0, j0, k0 + 1);
+        b = vol_at(vol, X, Y, Z, i0 + 1, j0, k0 + 1);
+        c10 = fmaf(wi, b - a, a);
+        a = vol_at(vol, X, Y, Z, i0, j0 + 1, k0 + 1);
+        b = vol_at(vol, X, Y, Z, i0 + 1, j0 + 1, k0 + 1);
+        c11 = fmaf(wi, b - a, a);
+    }
+    const float c0 = fmaf(wj, c01 - c00, c00);
+    const float c1 = fmaf(wj, c11 - c10, c10);
+    return fmaf(wk, c1 - c0, c0);
+}
+
+struct ForwardJob {
+    int geometry;
+    const float *vol;
+    int X, Y, Z;
+    double d, sod, odd, delta, rot, offset, h;
+    float *out;
+    int layout, nu, nv, n_views;
+};
+
+__global__ __launch_bounds__(256) void forward_kernel(ForwardJob J)
+{
+    const int iu = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int iv = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int n = blockIdx.z;
+    if (iu >= J.nu || iv >= J.nv) return;
+    // ray set-up in FP64 (FBP3D.cpp:280-284 angle convention)
+    double th = 2.0 * M_PI * J.rot * (double)n / (double)J.n_views + J.offset;
+    double s, c;
+    sincos(th, &s, &c);
+    double u = (iu + 0.5 - 0.5 * J.nu) * J.delta;
+    double v = (iv + 0.5 - 0.5 * J.nv) * J.delta;
+    double o[3], dir[3];
+    if (J.geometry == CTR_GEOMETRY_CONE) {
+        o[0] = J.sod * c;
+        o[1] = 0.0;
+        o[2] = J.sod * s;
+        dir[0] = -J.odd * c - u * s - o[0];
+        dir[1] = v;
+        dir[2] = -J.odd * s + u * c - o[2];
+        double len = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+        dir[0] /= len;
+        dir[1] /= len;
+        dir[2] /= len;
+    } else {
+        o[0] = -u * s;
+        o[1] = v;
+        o[2] = u * c;
+        dir[0] = -c;
+        dir[1] = 0.0;
+        dir[2] = -s;
+    }
+    const double half[3] = {(0.5 * J.X + 1.0) * J.d, (0.5 * J.Y + 1.0) * J.d,
+                            (0.5 * J.Z + 1.0) * J.d};
+    double tmin = -1e300, tmax = 1e300;
+    bool hit = true;
+    for (int a = 0; a < 3; a++) {
+        if (dir[a] == 0.0) {
+            if (o[a] < -half[a] || o[a] > half[a]) hit = false;
+            continue;
+        }
+        double t0 = (-half[a] - o[a]) / dir[a], t1 = (half[a] - o[a]) / dir[a];
+        if (t0 > t1) {
+            double t = t0;
+            t0 = t1;
+            t1 = t;
+        }
+        tmin = fmax(tmin, t0);
+        tmax = fmin(tmax, t1);
+    }
+    float acc = 0.0f;
+    if (hit && tmax > tmin) {
+        const int K = (int)ceil((tmax - tmin) / J.h);
+        // index-space ray: f(t') = base + t' * step with t' = k + 1/2
+        const double inv_d = 1.0 / J.d;
+        const float bi = (float)((o[0] + tmin * dir[0]) * inv_d + 0.5 * J.X - 0.5);
+        const float bj = (float)((o[1] + tmin * dir[1]) * inv_d + 0.5 * J.Y - 0.5);
+        const float bk = (float)((o[2] + tmin * dir[2]) * inv_d + 0.5 * J.Z - 0.5);
+        const float si = (float)(dir[0] * J.h * inv_d);
+        const float sj = (float)(dir[1] * J.h * inv_d);
+        const float sk = (float)(dir[2] * J.h * inv_d);
+        for (int k = 0; k < K; ++k) {
+            const float t = (float)k + 0.5f;
+            acc += trilinear0(J.vol, J.X, J.Y, J.Z, fmaf(t, si, bi), fmaf(t, sj, bj),
+                              fmaf(t, sk, bk));
+        }
+        acc *= (float)J.h;
+    }
+    size_t idx = J.layout == CTR_LAYOUT_PROJECTIONS
+                     ? ((size_t)n * J.nv + iv) * J.nu + iu
+                     : ((size_t)iv * J.n_views + n) * J.nu + iu;
+    J.out[idx] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Fan -> parallel rebinning (FBP2D::parallelize_fan_beam_sinogram)
+// ---------------------------------------------------------------------------
+__global__ void rebin_kernel(const float *fan, int nu_fan, int n_views, double sod, double sdd,
+                             double delta, double rot, float *par, int nu_par, double pitch_par)
+{
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)nu_par * n_views) return;
+    int is = (int)(idx % nu_par);
+    int n = (int)(idx / nu_par);
+    double dth = 2.0 * M_PI * rot / (double)n_views;
+    double theta = dth * (double)n;
+    double s = (is + 0.5 - 0.5 * nu_par) * pitch_par;
+    float out = 0.0f;
+    if (fabs(s) < sod) {
+        double gam = asin(s / sod);
+        double u = sdd * tan(gam);
+        double beta = theta + gam;
+        double fb = beta / dth;
+        double fu = u / delta + 0.5 * nu_fan - 0.5;
+        bool periodic = fabs(rot - 1.0) < 1e-12;
+        if (fu > -1.0 && fu < (double)nu_fan) {
+            double fb0 = floor(fb), fu0 = floor(fu);
+            double wb = fb - fb0, wu = fu - fu0;
+            double accum = 0.0;
+            for (int b = 0; b < 2; b++) {
+                long long vb = (long long)fb0 + b;
+                if (periodic) {
+                    vb %= n_views;
+                    if (vb < 0) vb += n_views;
+                } else if (vb < 0 || vb >= n_views) {
+                    continue;
+                }
+                double wbv = b ? wb : 1.0 - wb;
+                for (int a = 0; a < 2; a++) {
+                    int cu = (int)fu0 + a;
+                    if (cu < 0 || cu >= nu_fan) continue;
+                    double wa = a ? wu : 1.0 - wu;
+                    accum += wbv * wa * (double)fan[(size_t)vb * nu_fan + cu];
+                }
+            }
+            out = (float)accum;
+        }
+    }
+    par[idx] = out;
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise passes
+// ---------------------------------------------------------------------------
+__global__ void fdk_weight_kernel(float *proj, int layout, int nu, int nv, int n_views,
+                                  double sdd, double delta)
+{
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long total = (long long)nu * nv * n_views;
+    if (idx >= total) return;
+    int iu = (int)(idx % nu);
+    long long r = idx / nu;
+    int iv = layout == CTR_LAYOUT_PROJECTIONS ? (int)(r % nv) : (int)(r / n_views);
+    double u = (iu + 0.5 - 0.5 * nu) * delta, v = (iv + 0.5 - 0.5 * nv) * delta;
+    proj[idx] = (float)((double)proj[idx] * (sdd / sqrt(sdd * sdd + u * u + v * v)));
+}
+
+__global__ void log_normalize_kernel(float *p, long long count, float base, float floor_value)
+{
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= count) return;
+    p[idx] = -logf(fmaxf(p[idx], floor_value) / base);
+}
+
+__global__ void clip_kernel(float *p, long long count)
+{
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < count) p[idx] = fmaxf(p[idx], 0.0f);
+}
+
+__global__ void fill_kernel(float *p, long long count, float value)
+{
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < count) p[idx] = value;
+}
+
+// Ellipsoid record prepared on the host: centre (3), body-frame rows of
+// R^T scaled by 1/semi-axis (9), density (1).
+struct EllipsoidRec {
+    double c[3];
+    double m[9];
+    double density;
+};
+
+__global__ void render_kernel(const EllipsoidRec *E, int count, float *vol, int X, int Y, int Z,
+                              double d)
+{
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)X * Y * Z) return;
+    int i = (int)(idx % X);
+    int j = (int)((idx / X) % Y);
+    int k = (int)(idx / ((long long)X * Y));
+    double x = (i + 0.5 - 0.5 * X) * d, y = (j + 0.5 - 0.5 * Y) * d, z = (k + 0.5 - 0.5 * Z) * d;
+    float acc = vol[idx];
+    for (int e = 0; e < count; e++) {
+        const EllipsoidRec &R = E[e];
+        double p0 = x - R.c[0], p1 = y - R.c[1], p2 = z - R.c[2];
+        double q = 0.0;
+        for (int a = 0; a < 3; a++) {
+            double b = R.m[3 * a] * p0 + R.m[3 * a + 1] * p1 + R.m[3 * a + 2] * p2;
+            q += b * b;
+        }
+        if (q <= 1.0) acc += (float)R.density;
+    }
+    vol[idx] = acc;
+}
+
+// Counter-based generator: splitmix64 stream keyed by (seed, index).
+struct SplitMix {
+    unsigned long long s;
+    __device__ inline unsigned long long next()
+    {
+        unsigned long long z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    __device__ inline double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
+};
+
+// Poisson sample: multiplication method below mean 10, otherwise Hormann's
+// transformed rejection with squeeze (PTRS, 1993).  Loops are bounded.
+__device__ inline double poisson(SplitMix &g, double mu)
+{
+    if (mu <= 0.0) return 0.0;
+    if (mu < 10.0) {
+        double L = exp(-mu), p = 1.0;
+        int k = 0;
+        for (; k < 1000; ++k) {
+            p *= g.uniform();
+            if (p <= L) break;
+        }
+        return (double)k;
+    }
+    double smu = sqrt(mu), b = 0.931 + 2.53 * smu, a = -0.059 + 0.02483 * b;
+    double inv_alpha = 1.1239 + 1.1328 / (b - 3.4), vr = 0.9277 - 3.6224 / (b - 2.0);
+    double lmu = log(mu);
+    for (int it = 0; it < 256; ++it) {
+        double U = g.uniform() - 0.5, V = g.uniform();
+        double us = 0.5 - fabs(U);
+        double k = floor((2.0 * a / us + b) * U + mu + 0.43);
+        if (us >= 0.07 && V <= vr) return k;
+        if (k < 0.0 || (us < 0.013 && V > us)) continue;
+        if (log(V) + log(inv_alpha) - log(a / (us * us) + b) <= -mu + k * lmu - lgamma(k + 1.0))
+            return k;
+    }
+    return floor(mu + 0.5);
+}
+
+__global__ void poisson_log_kernel(float *p, long long count, double i0, unsigned long long seed)
+{
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= count) return;
+    SplitMix g{seed ^ ((unsigned long long)idx * 0xD1B54A32D192ED03ull)};
+    g.next();
+    double mu = i0 * exp(-(double)p[idx]);
+    double k = poisson(g, mu);
+    p[idx] = (float)(-log(fmax(k, 1.0) / i0));
+}
+
+}  // namespace ctr
+
+using namespace ctr;
+
+extern "C" {
+
+int ctr_forward_project(void *stream, int geometry, const float *volume, int X, int Y, int Z,
+                        double source_distance, double detector_distance,
+                        double detector_width, double volume_width, double rotation_count,
+                        double rotation_offset, float *out, int layout, int nu, int nv,
+                        int n_views, double step)
+{
+    clear_error();
+    CTR_REQUIRE(volume && out, CTR_ERR_NULL_POINTER,
+                "[CTAnalyzer Error] ctr_forward_project: volume or output is NULL");
+    CTR_REQUIRE(layout == CTR_LAYOUT_PROJECTIONS || layout == CTR_LAYOUT_SINOGRAMS,
+                CTR_ERR_INVALID_ARGUMENT, "unknown layout %d", layout);
+    CTR_REQUIRE(step > 0, CTR_ERR_INVALID_ARGUMENT, "step must be positive");
+    int rc = check_geometry(geometry, source_distance, detector_distance, detector_width,
+                            volume_width, X, Y, Z, nu, nv, n_views);
+    if (rc) return rc;
+    ForwardJob J;
+    J.geometry = geometry;
+    J.vol = volume;
+    J.X = X;
+    J.Y = Y;
+    J.Z = Z;
+    J.d = volume_width / X;
+    J.sod = source_distance;
+    J.odd = detector_distance;
+    J.delta = detector_width / nu;
+    J.rot = rotation_count;
+    J.offset = rotation_offset;
+    J.h = step * J.d;
+    J.out = out;
+    J.layout = layout;
+    J.nu = nu;
+    J.nv = nv;
+    J.n_views = n_views;
+    dim3 grid((nu + 15) / 16, (nv + 15) / 16, n_views);
+    forward_kernel<<<grid, 256, 0, as_stream(stream)>>>(J);
+    CTR_CHECK_LAUNCH();
+    return CTR_OK;
+}
+
+int ctr_rebin_fan_to_parallel(void *stream, const float *fan, int nu_fan, int n_views,
+                              double source_distance, double detector_distance,
+                              double detector_width, double rotation_count, float *parallel,
+                              int nu_par, double parallel_width)
+{
+    clear_error();
+    CTR_REQUIRE(fan && parallel, CTR_ERR_NULL_POINTER, "ctr_rebin_fan_to_parallel: NULL buffer");
+    CTR_REQUIRE(nu_fan > 0 && n_views > 0 && nu_par > 0, CTR_ERR_INVALID_ARGUMENT,
+                "ctr_rebin_fan_to_parallel: sizes must be positive");
+    CTR_REQUIRE(source_distance > 0 && detector_distance >= 0 && detector_width > 0 &&
+                    parallel_width > 0 && rotation_count != 0.0,
+                CTR_ERR_INVALID_ARGUMENT, "ctr_rebin_fan_to_parallel: bad geometry");
+    long long total = (long long)nu_par * n_views;
+    rebin_kernel<<<grid1d(total, 256), 256, 0, as_stream(stream)>>>(
+        fan, nu_fan, n_views, source_distance, source_distance + detector_distance,
+        detector_width / nu_fan, rotation_count, parallel, nu_par, parallel_width / nu_par);
+    CTR_CHECK_LAUNCH();
+    return CTR_OK;
+}
+
+int ctr_fdk_weight(void *stream, float *proj, int layout, int nu, int nv, int n_views,
+                   double source_distance, double detector_distance, double detector_width)
+{
+    clear_error();
+    CTR_REQUIRE(proj, CTR_ERR_NULL_POINTER, "ctr_fdk_weight: projections are NULL");
+    CTR_REQUIRE(nu > 0 && nv > 0 && n_views >= 0, CTR_ERR_INVALID_ARGUMENT,
+                "ctr_fdk_weight: bad sizes");
+    CTR_REQUIRE(layout == CTR_LAYOUT_PROJECTIONS || layout == CTR_LAYOUT_SINOGRAMS,
+                CTR_ERR_INVALID_ARGUMENT, "unknown layout %d", layout);
+    CTR_REQUIRE(source_distance > 0 && detector_distance >= 0 && detector_width > 0,
+                CTR_ERR_INVALID_ARGUMENT, "ctr_fdk_weight: bad geometry");
+    long long total = (long long)nu * nv * n_views;
+    if (total == 0) return CTR_OK;
+    fdk_weight_kernel<<<grid1d(total, 256), 256, 0, as_stream(stream)>>>(
+        proj, layout, nu, nv, n_views, source_distance + detector_distance,
+        detector_width / nu);
+    CTR_CHECK_LAUNCH();
+    return CTR_OK;
+}
+
+int ctr_log_normalize(void *stream, float *data, size_t count, float base, float floor_value)
+{
+    clear_error();
+    CTR_REQUIRE(data || count == 0, CTR_ERR_NULL_POINTER, "ctr_log_normalize: NULL buffer");
+    CTR_REQUIRE(base > 0, CTR_ERR_INVALID_ARGUMENT, "ctr_log_normalize: base must be > 0");
+    if (count == 0) return CTR_OK;
+    log_normalize_kernel<<<grid1d((long long)count, 256), 256, 0, as_stream(stream)>>>(
+        data, (long long)count, base, floor_value);
+    CTR_CHECK_LAUNCH();
+    return CTR_OK;
+}
+
+int ctr_clip_negative(void *stream, float *data, size_t count)
+{
+    clear_error();
+    CTR_REQUIRE(data || count == 0, CTR_ERR_NULL_POINTER, "ctr_clip_negative: NULL buffer");
+    if (count == 0) return CTR_OK;
+    clip_kernel<<<grid1d((long long)count, 256), 256, 0, as_stream(stream)>>>(data,
+                                                                             (long long)count);
+    CTR_CHECK_LAUNCH();
+    return CTR_OK;
+}
+
+int ctr_fill(void *stream, float *data, size_t count, float value)
+{
+    clear_error();
+    CTR_REQUIRE(data || count == 0, CTR_ERR_NULL_POINTER, "ctr_fill: NULL buffer");
+    if (count == 0) return CTR_OK;
+    fill_kernel<<<grid1d((long long)count, 256), 256, 0, as_stream(stream)>>>(
+        data, (long long)count, value);
+    CTR_CHECK_LAUNCH();
+    return CTR_OK;
+}
+
+int ctr_render_ellipsoids(void *stream, ctr_workspace *ws, const double *ellipsoids, int count,
+                          float *volume, int X, int Y, int Z, double volume_width)
+{
+    clear_error();
+    CTR_REQUIRE(volume && (ellipsoids || count == 0), CTR_ERR_NULL_POINTER,
+                "[CTAnalyzer Error] ctr_render_ellipsoids: NULL buffer");
+    CTR_REQUIRE(count >= 0 && count <= 4096, CTR_ERR_INVALID_ARGUMENT,
+                "ctr_render_ellipsoids: count %d outside [0, 4096]", count);
+    CTR_REQUIRE(X > 0 && Y > 0 && Z > 0 && volume_width > 0, CTR_ERR_INVALID_ARGUMENT,
+                "ctr_render_ellipsoids: bad volume");
+    if (count == 0) return CTR_OK;
+    std::vector<EllipsoidRec> recs(count);
+    for (int e = 0; e < count; e++) {
+        const double *P = ellipsoids + 10 * (size_t)e;
+        CTR_REQUIRE(P[3] > 0 && P[4] > 0 && P[5] > 0, CTR_ERR_INVALID_ARGUMENT,
+                    "ellipsoid %d has a non-positive semi-axis", e);
+        double ca = cos(P[6]), sa = sin(P[6]), cb = cos(P[7]), sb = sin(P[7]);
+        double cg = cos(P[8]), sg = sin(P[8]);
+        // R = Rz(g) Ry(b) Rx(a)
+        double Rx[3][3] = {{1, 0, 0}, {0, ca, -sa}, {0, sa, ca}};
+        double Ry[3][3] = {{cb, 0, sb}, {0, 1, 0}, {-sb, 0, cb}};
+        double Rz[3][3] = {{cg, -sg, 0}, {sg, cg, 0}, {0, 0, 1}};
+        double T[3][3], R[3][3];
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) {
+                T[r][c] = 0;
+                for (int m = 0; m < 3; m++) T[r][c] += Ry[r][m] * Rx[m][c];
+            }
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) {
+                R[r][c] = 0;
+                for (int m = 0; m < 3; m++) R[r][c] += Rz[r][m] * T[m][c];
+            }
+        EllipsoidRec &Q = recs[e];
+        for (int a = 0; a < 3; a++) {
+            Q.c[a] = P[a];
+            for (int b = 0; b < 3; b++) Q.m[3 * a + b] = R[b][a] / P[3 + a];
+        }
+        Q.density = P[9];
+    }
+    ctr_workspace *w = nullptr;
+    int rc = workspace_resolve(ws, &w);
+    if (rc) return rc;
+    void *p = nullptr;
+    if ((rc = workspace_get(w, SLOT_ELLIPSOIDS, sizeof(EllipsoidRec) * count, &p))) return rc;
+    hipStream_t s = as_stream(stream);
+    CTR_TRY_HIP(hipMemcpyAsync(p, recs.data(), sizeof(EllipsoidRec) * count,
+                               hipMemcpyHostToDevice, s));
+    long long total = (long long)X * Y * Z;
+    render_kernel<<<grid1d(total, 256), 256, 0, s>>>((const EllipsoidRec *)p, count, volume, X,
+                                                     Y, Z, volume_width / X);
+    CTR_CHECK_LAUNCH();
+    // the host records must outlive the copy
+    CTR_TRY_HIP(hipStreamSynchronize(s));
+    return CTR_OK;
+}
+
+int ctr_poisson_log(void *stream, float *data, size_t count, double i0, uint64_t seed)
+{
+    clear_error();
+    CTR_REQUIRE(data || count == 0, CTR_ERR_NULL_POINTER, "ctr_poisson_log: NULL buffer");
+    CTR_REQUIRE(i0 > 0, CTR_ERR_INVALID_ARGUMENT, "ctr_poisson_log: i0 must be > 0");
+    if (count == 0) return CTR_OK;
+    poisson_log_kernel<<<grid1d((long long)count, 256), 256, 0, as_stream(stream)>>>(
+        data, (long long)count, i0, (unsigned long long)seed);
+    CTR_CHECK_LAUNCH();
+    return CTR_OK;
+}
+
+}  // extern "C"

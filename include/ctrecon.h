@@ -1,0 +1,270 @@
+/*
+ * ctrecon.h -- C ABI of the MI355X (gfx950) CT reconstruction library.
+ *
+ * This is the drop-in boundary under the C++ classes FBP2D, FBP3D and FFFT
+ * (the reference's Source/FilteredBackProjection/FBP2D.h, FBP3D.h and
+ * Source/FastFourierTransformGPU/FastFourierTransformGPU.h).  Every entry
+ * point below names the reference interface it replaces.  Signatures hold
+ * only plain C types: device pointers are `float*` / `void*` into HIP device
+ * memory owned by the caller, streams are `void*` (a hipStream_t, or NULL for
+ * the default stream), geometry is plain doubles with the reference's
+ * parameter meaning.  All calls are asynchronous on the given stream unless
+ * stated otherwise.
+ *
+ * Return value: CTR_OK (0) or a negative CTR_ERR_* code; ctr_last_error()
+ * returns the calling thread's last message.
+ *
+ * Conventions (shared with the CPU oracle):
+ *   view angle    theta_n = 2*pi*rotation_count*n/N + rotation_offset
+ *   volume        (X, Y, Z), x fastest, Y = rotation axis, pitch
+ *                 d = volume_width / X; voxel centre (i + 1/2 - X/2) d, ...
+ *   projections   layout CTR_LAYOUT_PROJECTIONS = (nu, nv, N), u fastest;
+ *                 layout CTR_LAYOUT_SINOGRAMS   = (nu, N, nv)
+ *   detector      pitch delta = detector_width / nu (square pixels);
+ *                 pixel centre u = (iu + 1/2 - nu/2) delta
+ *   cone          source at SOD (cos t, 0, sin t); detector plane at
+ *                 detector_distance beyond the axis; SDD = SOD + that
+ *   2D            an image (W, W) is a volume (W, 1, W); a sinogram (nu, N)
+ *                 is projections with nv = 1
+ */
+#ifndef CTRECON_H
+#define CTRECON_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CTR_OK 0
+#define CTR_ERR_INVALID_ARGUMENT (-1)
+#define CTR_ERR_NOT_POWER_OF_TWO (-2)
+#define CTR_ERR_OUT_OF_MEMORY (-3)
+#define CTR_ERR_HIP (-4)
+#define CTR_ERR_NULL_POINTER (-5)
+#define CTR_ERR_OVER_BUDGET (-6)
+
+/* element types of staged (filtered) projections */
+#define CTR_F32 0
+#define CTR_F16 1
+#define CTR_BF16 2
+
+/* FBP2D::FilterType (Source/FilteredBackProjection/FBP2D.h:10-14) */
+#define CTR_RAM_LAK 0
+#define CTR_SHEPP_LOGAN 1
+#define CTR_COSINE 2
+
+#define CTR_GEOMETRY_CONE 0
+#define CTR_GEOMETRY_PARALLEL 1
+
+#define CTR_LAYOUT_PROJECTIONS 0
+#define CTR_LAYOUT_SINOGRAMS 1
+
+/* blit modes of FFFT::blit_texture_* (FastFourierTransformGPU.cpp:144-186) */
+#define CTR_BLIT_REAL_TO_COMPLEX 0
+#define CTR_BLIT_COMPLEX_TO_COMPLEX 1
+#define CTR_BLIT_COMPLEX_TO_REAL_R 2
+#define CTR_BLIT_COMPLEX_TO_REAL_I 3
+
+typedef struct ctr_workspace ctr_workspace;
+
+/* ---- library ------------------------------------------------------------ */
+/* Thread-local text of the last error ("" if none).  Stands in for the
+ * reference's "[CTAnalyzer Error] ..." console messages (e.g.
+ * Source/FilteredBackProjection/FBP3D.cpp:214-217). */
+const char *ctr_last_error(void);
+int ctr_version(void);
+/* Waits for all work on the device (no reference counterpart; the GL path
+ * synchronised implicitly). */
+int ctr_synchronize(void);
+
+/* ---- workspace (scratch owned by the library) ------------------------------
+ * Per-view geometry tables, FFT twiddles and per-view maxima live here so
+ * that callers never see allocations inside hot loops.  Passing NULL to any
+ * entry point uses a per-device default workspace.  A workspace grows on
+ * demand (hipMalloc, synchronous); ctr_workspace_reserve pre-sizes it for
+ * the given problem so that later calls allocate nothing.  No reference
+ * counterpart: the reference allocated textures per call
+ * (FastFourierTransformGPU.cpp:15,74,115,138). */
+int ctr_workspace_create(ctr_workspace **out);
+int ctr_workspace_destroy(ctr_workspace *ws);
+int ctr_workspace_reserve(ctr_workspace *ws, int n_views, int max_fft_length);
+
+/* ---- FFT family: FFFT (Source/FastFourierTransformGPU/) --------------------
+ * Complex data are interleaved float pairs (RG32F texture), rows of n
+ * complex values, `rows` rows. */
+
+/* FFFT::fft_radix2 (FastFourierTransformGPU.cpp:3-28): in-place forward FFT
+ * of every row, exp(-2 pi i k/n) kernel, unnormalised.  n must be a power of
+ * two in [1, 8192] (CTR_ERR_NOT_POWER_OF_TWO otherwise, as at :10-13). */
+int ctr_fft_radix2(void *stream, float *data, int n, int rows);
+/* FFFT::inverse_fft_radix2 (FastFourierTransformGPU.cpp:30-36):
+ * conj -> FFT -> conj -> divide by n, in place. */
+int ctr_inverse_fft_radix2(void *stream, float *data, int n, int rows);
+/* FFFT::fft_shift(src, dst, shift) (FastFourierTransformGPU.cpp:103-111):
+ * dst[(i + shift) mod n] = src[i] per row; src == dst is allowed (the
+ * reference's in-place overload, :113-119). */
+int ctr_fft_shift(void *stream, const float *src, float *dst, int n, int rows,
+                  int shift);
+/* FFFT::blit_texture_* (FastFourierTransformGPU.cpp:144-186): copies the
+ * min(src, dst) extent; real textures hold 1 float per texel, complex 2.
+ * Texels of dst outside the copied extent are left untouched. */
+int ctr_blit(void *stream, int mode, const float *src, int src_width,
+             int src_height, float *dst, int dst_width, int dst_height);
+/* FFFT::conjugate_complex_texture / multiply_complex_texture /
+ * divide_complex_texture (FastFourierTransformGPU.cpp:212-236); count is
+ * the number of complex values. */
+int ctr_conjugate(void *stream, float *data, size_t count);
+int ctr_scale_complex(void *stream, float *data, size_t count,
+                      float multiplier);
+
+/* FFFT::reverse_bit_swap (FastFourierTransformGPU.cpp:238-244): in-place
+ * bit-reversal permutation of every row (n a power of two). */
+int ctr_fft_bit_reverse(void *stream, float *data, int n, int rows);
+/* FFFT::fft_single_step (FastFourierTransformGPU.cpp:246-256): one radix-2
+ * DIT stage of span 2 << step_index from `read` into `write` (distinct
+ * buffers), one output per element, as in the reference's CPU guide
+ * (Source/FastFourierTransformCPU/ParallelFastFourierTransform.cpp:274-290). */
+int ctr_fft_single_stage(void *stream, const float *read, float *write, int n, int rows,
+                         int step_index);
+/* Real-texture scaling, as FBP2D::_int_texture_div divides by `denominator`
+ * (FBP2D.cpp:189-195): data[i] *= multiplier. */
+int ctr_scale_real(void *stream, float *data, size_t count, float multiplier);
+
+/* FBP2D::apply_filter (Source/FilteredBackProjection/FBP2D.cpp:124-144):
+ * multiplies the fft-SHIFTED spectrum rows in place by
+ * H[i] = n * g(i/n - 1/2), g = |x| (RAM_LAK), sin|x| (SHEPP_LOGAN),
+ * |sin(pi x)|/pi (COSINE) (FastFourierTransformCPU/
+ * FastFourierTransformImage.cpp:35-37,45).  CTR_ERR_INVALID_ARGUMENT for an
+ * unknown filter (FBP2D.cpp:135-137). */
+int ctr_apply_filter(void *stream, float *spectrum, int n, int rows,
+                     int filter);
+
+/* ---- fused row filter ------------------------------------------------------
+ * The whole per-row chain of FBP3D::apply_filter_to_projections /
+ * apply_filter_to_sinograms (FBP3D.cpp:389-442) in one LDS-resident pass:
+ * zero-pad to fft_length, FFT, x H (unshifted index form of the shifted
+ * gain), inverse FFT, crop the first `width` samples, real part.  Two real
+ * rows share one complex FFT (H is real and even).  fft_length 0 selects the
+ * reference default 2^ceil(log2 width) (FastFourierTransformGPU.cpp:258-280);
+ * any power of two >= width up to 8192 is accepted.  in may equal out. */
+int ctr_filter_rows(void *stream, const float *in, float *out, int width,
+                    int rows, int fft_length, int filter);
+
+/* FDK pre-weight and ramp filter of all projections, written in the staged
+ * layout the backprojector reads (below).  fdk_weight != 0 multiplies each
+ * sample by SDD / sqrt(SDD^2 + u^2 + v^2) (FP64) before filtering
+ * (FBP3D.cpp:369-387).  For CTR_F16 / CTR_BF16 every view is scaled by its
+ * max |value| (stored in view_scale[N], device memory) before rounding; for
+ * CTR_F32 view_scale may be NULL.  `layout` gives the input layout. */
+int ctr_filter_projections(void *stream, ctr_workspace *ws, const float *proj,
+                           int layout, int nu, int nv, int n_views,
+                           int fft_length, int filter, int fdk_weight,
+                           double source_distance, double detector_distance,
+                           double detector_width, void *staged,
+                           int staged_dtype, float *view_scale);
+
+/* ---- staged projections ------------------------------------------------------
+ * View-major, zero-bordered copy of the filtered projections that the
+ * backprojector samples.  For CTR_F32 texel (iu, iv, n) is
+ *   staged[(n * pitch_v + iv + 1) * pitch_u + iu + 1],
+ * with one zero column/row before and two after the data.  For CTR_F16 and
+ * CTR_BF16 each 32-bit texel holds the pair (q[iu], q[iu+1]) / view_scale[n]
+ * so one aligned load feeds both horizontal taps.  No reference counterpart
+ * (the reference sampled its Texture2DArray directly, FBP3D.cpp:543). */
+int ctr_staged_layout(int nu, int nv, int n_views, int dtype, int *pitch_u,
+                      int *pitch_v, size_t *bytes);
+/* Copies plain projections (either layout) into the staged layout,
+ * computing the per-view scale for CTR_F16 / CTR_BF16. */
+int ctr_stage_projections(void *stream, ctr_workspace *ws, const float *proj,
+                          int layout, int nu, int nv, int n_views,
+                          void *staged, int staged_dtype, float *view_scale);
+
+/* ---- backprojection ----------------------------------------------------------
+ * Voxel-driven backprojection over all views in ONE launch (the reference
+ * issues one full-volume read-modify-write dispatch per view,
+ * FBP3D.cpp:538-556).  Replaces
+ *   FBP3D::project_backward_cone_fdk_from_projections   (FBP3D.cpp:524-559)
+ *   FBP3D::project_backward_cone_fdk_from_sinograms     (FBP3D.cpp:561-593)
+ *   FBP3D::project_backward_parallel_from_projections   (FBP3D.cpp:460-490)
+ *   FBP3D::project_backward_parallel_from_sinograms     (FBP3D.cpp:492-522)
+ *   FBP2D::project_backward                             (FBP2D.cpp:96-122)
+ * For every voxel:
+ *   vol = (accumulate ? vol : 0)
+ *       + scale * sum_n view_scale[n] * w_n^2 * bilinear(staged_n, fu, fv)
+ * with w = SOD / (SOD - depth) for cone geometry and 1 for parallel;
+ * bilinear is zero outside the detector.  clip_negative != 0 then applies
+ * max(v, 0) (FBP3D.cpp:444-458, run after every reference 3D
+ * backprojection).  view_scale may be NULL for CTR_F32. */
+int ctr_backproject(void *stream, ctr_workspace *ws, int geometry,
+                    const void *staged, int staged_dtype,
+                    const float *view_scale, int nu, int nv, int n_views,
+                    double source_distance, double detector_distance,
+                    double detector_width, double volume_width,
+                    double rotation_count, double rotation_offset,
+                    float *volume, int X, int Y, int Z, float scale,
+                    int accumulate, int clip_negative);
+
+/* ---- forward projection ------------------------------------------------------
+ * Ray-driven projector, one ray per detector pixel centre, replacing
+ *   FBP3D::project_forward_cone_to_projections   (FBP3D.cpp:269-299)
+ *   FBP3D::project_forward_cone_to_sinogram      (FBP3D.cpp:301-330)
+ *   FBP3D::project_forward_parallel_to_projections (FBP3D.cpp:212-240)
+ *   FBP3D::project_forward_parallel_to_sinogram  (FBP3D.cpp:242-267)
+ *   FBP2D::project_forward / project_forward_equiangular (FBP2D.cpp:54-94)
+ * The ray is clipped to the volume box grown by one voxel per side; samples
+ * at t_in + (k + 1/2) h, h = step * d, trilinear (zero outside), times h.
+ * Output overwrites `out` in the given layout. */
+int ctr_forward_project(void *stream, int geometry, const float *volume,
+                        int X, int Y, int Z, double source_distance,
+                        double detector_distance, double detector_width,
+                        double volume_width, double rotation_count,
+                        double rotation_offset, float *out, int layout,
+                        int nu, int nv, int n_views, double step);
+
+/* ---- elementwise detector / volume passes ------------------------------------ */
+/* FBP3D::apply_fdk_weights_to_projections (FBP3D.cpp:369-387), in place,
+ * either layout; weight SDD / sqrt(SDD^2 + u^2 + v^2) evaluated in FP64. */
+int ctr_fdk_weight(void *stream, float *proj, int layout, int nu, int nv,
+                   int n_views, double source_distance,
+                   double detector_distance, double detector_width);
+/* FBP3D::log_normalize_projections (FBP3D.cpp:333-349), in place:
+ * p = -ln(max(I, floor_value) / base). */
+int ctr_log_normalize(void *stream, float *data, size_t count, float base,
+                      float floor_value);
+/* FBP3D::clip_negatives_of_volume (FBP3D.cpp:444-458): v = max(v, 0). */
+int ctr_clip_negative(void *stream, float *data, size_t count);
+/* Texture*::clear(value) as used by generate_blank_* (FBP3D.cpp:16-56). */
+int ctr_fill(void *stream, float *data, size_t count, float value);
+
+/* FBP2D::parallelize_fan_beam_sinogram (FBP2D.cpp:31-52): resamples a flat-
+ * detector fan sinogram (nu_fan, N) over 2*pi*rotation_count into a parallel
+ * sinogram (nu_par, N) over the same angles with detector pitch
+ * parallel_width / nu_par, by bilinear interpolation in (fan angle,
+ * detector) space; parallel rays outside the fan are 0. */
+int ctr_rebin_fan_to_parallel(void *stream, const float *fan, int nu_fan,
+                              int n_views, double source_distance,
+                              double detector_distance, double detector_width,
+                              double rotation_count, float *parallel,
+                              int nu_par, double parallel_width);
+
+/* ---- test-input generators -------------------------------------------------- */
+/* FBP3D::render_ellipsoid (FBP3D.cpp:609-628): adds density to every voxel
+ * whose centre lies inside.  ellipsoids: host array of `count` records of 10
+ * doubles (cx cy cz a b c alpha beta gamma density) in world units, body-to-
+ * world rotation Rz(gamma) Ry(beta) Rx(alpha).  count <= 4096. */
+int ctr_render_ellipsoids(void *stream, ctr_workspace *ws,
+                          const double *ellipsoids, int count, float *volume,
+                          int X, int Y, int Z, double volume_width);
+/* Replaces every line integral p by -ln(max(K, 1) / i0) with
+ * K ~ Poisson(i0 * exp(-p)) from a counter-based generator keyed by
+ * (seed, element index): the noise model of the benchmark's cone config.
+ * No reference counterpart. */
+int ctr_poisson_log(void *stream, float *data, size_t count, double i0,
+                    uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

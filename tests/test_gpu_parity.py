@@ -1,0 +1,209 @@
+"""HIP path vs the FP64 CPU oracle, through the C ABI, on seeded inputs.
+
+Tolerances: FP32 kernels <= 1e-4 max relative error (max |gpu - oracle| /
+max |oracle|); FP16-stored projections <= 1e-3 relative RMSE; BF16-stored
+projections <= 4e-3 relative RMSE (8-bit mantissa, reported alongside).
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_max, rel_rmse
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(dev, a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(dev)
+
+
+@pytest.mark.parametrize("n", [1, 2, 8, 64, 512, 2048, 8192])
+def test_fft_rows_vs_oracle(ctr, orc, dev, n):
+    import torch
+    rng = np.random.default_rng(n)
+    rows = 5
+    z = (rng.standard_normal((rows, n)) + 1j * rng.standard_normal((rows, n))).astype(np.complex64)
+    t = torch.from_numpy(z).to(dev)
+    ctr.fft_radix2(t)
+    got = t.cpu().numpy()
+    ref = np.stack([orc.fft_radix2(r) for r in z.astype(np.complex128)])
+    assert rel_max(got, ref) < 2e-6 * max(1, np.log2(n))
+    ctr.fft_radix2(t, inverse=True)
+    assert rel_max(t.cpu().numpy(), z) < 1e-5
+
+
+def test_fft_rejects_non_power_of_two(ctr, dev):
+    import torch
+    t = torch.zeros((2, 12), dtype=torch.complex64, device=dev)
+    with pytest.raises(ctr.CtrError) as e:
+        ctr.fft_radix2(t)
+    assert e.value.code == -2
+
+
+@pytest.mark.parametrize("width,npad,ftype", [
+    (367, 0, 0), (367, 1024, 2), (768, 2048, 0), (768, 2048, 1), (768, 2048, 2),
+    (100, 128, 0), (1, 1, 0), (5, 8, 2), (1536, 4096, 0)])
+def test_filter_rows_vs_oracle(ctr, orc, dev, width, npad, ftype):
+    rng = np.random.default_rng(width + npad)
+    rows = 7
+    x = rng.standard_normal((rows, width)).astype(np.float32)
+    got = ctr.filter_rows(_t(dev, x), ftype, npad, out=_t(dev, np.zeros_like(x))).cpu().numpy()
+    ref = orc.filter_rows(x, ftype, npad or None)
+    assert rel_max(got, ref) < 1e-5
+
+
+def test_filter_rows_in_place_odd_rows(ctr, orc, dev):
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((9, 200)).astype(np.float32)
+    t = _t(dev, x)
+    ctr.filter_rows(t, 2, 512)
+    assert rel_max(t.cpu().numpy(), orc.filter_rows(x, 2, 512)) < 1e-5
+
+
+def _cone(ctr, **kw):
+    base = dict(kind=ctr.CONE, nu=64, nv=48, n_views=90, X=40, Y=32, Z=36,
+                detector_width=128.0, volume_width=40.0, source_distance=150.0,
+                detector_distance=100.0)
+    base.update(kw)
+    return ctr.Geometry(**base)
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_filter_projections_fdk_vs_oracle(ctr, orc, dev, layout):
+    import torch
+    g = _cone(ctr)
+    rng = np.random.default_rng(11)
+    proj = rng.standard_normal((g.n_views, g.nv, g.nu)).astype(np.float32)
+    src = proj if layout == 0 else np.ascontiguousarray(proj.transpose(1, 0, 2))
+    staged, _ = ctr.filter_projections(_t(dev, src), g, 0, 256, ctr.F32, fdk=True,
+                                       layout=layout)
+    pu, pv, _ = ctr.staged_layout(g.nu, g.nv, g.n_views)
+    st = staged.view(torch.float32).cpu().numpy().reshape(g.n_views, pv, pu)
+    assert np.all(st[:, 0, :] == 0) and np.all(st[:, g.nv + 1:, :] == 0)
+    assert np.all(st[:, :, 0] == 0) and np.all(st[:, :, g.nu + 1:] == 0)
+    got = st[:, 1:g.nv + 1, 1:g.nu + 1]
+    w = orc.fdk_weight(proj, g.source_distance, g.detector_distance, g.detector_width)
+    ref = orc.filter_rows(w.astype(np.float32), 0, 256)
+    assert rel_max(got, ref) < 1e-5
+
+
+def _fdk_ref(orc, g, q, scale):
+    return orc.backproject_cone(q, g.source_distance, g.detector_distance,
+                                g.detector_width, g.volume_width, g.rotation_count,
+                                g.rotation_offset, scale, (g.X, g.Y, g.Z))
+
+
+@pytest.mark.parametrize("shape", [(40, 32, 36), (17, 9, 23), (64, 64, 64)])
+def test_backproject_cone_f32_vs_oracle(ctr, orc, dev, shape):
+    X, Y, Z = shape
+    g = _cone(ctr, X=X, Y=Y, Z=Z, rotation_offset=0.3)
+    rng = np.random.default_rng(X)
+    q = rng.standard_normal((g.n_views, g.nv, g.nu)).astype(np.float32)
+    staged, _ = ctr.stage_projections(_t(dev, q), g, ctr.F32)
+    got = ctr.backproject(staged, g, ctr.F32, scale=0.01).cpu().numpy()
+    ref = _fdk_ref(orc, g, q, 0.01)
+    assert rel_max(got, ref) < 1e-4
+
+
+def test_backproject_accumulate_and_clip(ctr, orc, dev):
+    import torch
+    g = _cone(ctr)
+    rng = np.random.default_rng(5)
+    q = rng.standard_normal((g.n_views, g.nv, g.nu)).astype(np.float32)
+    staged, _ = ctr.stage_projections(_t(dev, q), g, ctr.F32)
+    base = torch.full((g.Z, g.Y, g.X), 0.25, device=dev)
+    got = ctr.backproject(staged, g, ctr.F32, scale=0.01, volume=base, accumulate=True,
+                          clip=True).cpu().numpy()
+    ref = np.maximum(_fdk_ref(orc, g, q, 0.01) + 0.25, 0.0)
+    assert rel_max(got, ref) < 1e-4
+    assert got.min() >= 0.0
+
+
+@pytest.mark.parametrize("dtype,tol", [(1, 1e-3), (2, 4e-3)])
+def test_backproject_16bit_storage(ctr, orc, dev, dtype, tol):
+    g = _cone(ctr)
+    rng = np.random.default_rng(21)
+    p = rng.standard_normal((g.n_views, g.nv, g.nu)).astype(np.float32)
+    nfft = 256
+    staged, vs = ctr.filter_projections(_t(dev, p), g, 0, nfft, dtype, fdk=True)
+    assert vs is not None
+    got = ctr.backproject(staged, g, dtype, view_scale=vs, scale=g.fbp_scale(nfft)).cpu().numpy()
+    w = orc.fdk_weight(p, g.source_distance, g.detector_distance, g.detector_width)
+    q = orc.filter_rows(w.astype(np.float32), 0, nfft).astype(np.float32)
+    ref = _fdk_ref(orc, g, q, g.fbp_scale(nfft))
+    assert rel_rmse(got, ref) < tol
+
+
+def test_backproject_parallel_vs_oracle(ctr, orc, dev):
+    g = ctr.Geometry(ctr.PARALLEL, nu=50, nv=20, n_views=64, X=32, Y=16, Z=30,
+                     detector_width=60.0, volume_width=40.0, rotation_count=0.5,
+                     rotation_offset=0.1)
+    rng = np.random.default_rng(9)
+    q = rng.standard_normal((g.n_views, g.nv, g.nu)).astype(np.float32)
+    staged, _ = ctr.stage_projections(_t(dev, q), g, ctr.F32)
+    got = ctr.backproject(staged, g, ctr.F32, scale=0.02).cpu().numpy()
+    ref = orc.backproject_parallel(q, g.detector_width, g.volume_width, g.rotation_count,
+                                   g.rotation_offset, 0.02, (g.X, g.Y, g.Z))
+    assert rel_max(got, ref) < 1e-4
+
+
+def test_sinogram_layout_matches_projection_layout(ctr, dev):
+    g = _cone(ctr)
+    rng = np.random.default_rng(2)
+    q = rng.standard_normal((g.n_views, g.nv, g.nu)).astype(np.float32)
+    a, _ = ctr.stage_projections(_t(dev, q), g, ctr.F32, layout=0)
+    b, _ = ctr.stage_projections(_t(dev, q.transpose(1, 0, 2)), g, ctr.F32, layout=1)
+    va = ctr.backproject(a, g, ctr.F32).cpu().numpy()
+    vb = ctr.backproject(b, g, ctr.F32).cpu().numpy()
+    assert np.array_equal(va, vb)
+
+
+@pytest.mark.parametrize("kind", ["cone", "parallel"])
+def test_forward_project_vs_oracle(ctr, orc, dev, kind):
+    import torch
+    rng = np.random.default_rng(4)
+    X, Y, Z = 24, 20, 28
+    vol = rng.random((Z, Y, X)).astype(np.float32)
+    if kind == "cone":
+        g = ctr.Geometry(ctr.CONE, nu=40, nv=36, n_views=24, X=X, Y=Y, Z=Z,
+                         detector_width=80.0, volume_width=24.0, source_distance=60.0,
+                         detector_distance=40.0, rotation_offset=0.2)
+        ref = orc.forward_cone(vol, g.volume_width, g.source_distance, g.detector_distance,
+                               g.detector_width, g.nu, g.nv, g.n_views, 1.0, 0.2, 0.5)
+    else:
+        g = ctr.Geometry(ctr.PARALLEL, nu=40, nv=24, n_views=24, X=X, Y=Y, Z=Z,
+                         detector_width=40.0, volume_width=24.0, rotation_count=0.5)
+        ref = orc.forward_parallel(vol, g.volume_width, g.detector_width, g.nu, g.nv,
+                                   g.n_views, 0.5, 0.0, 0.5)
+    got = ctr.forward_project(_t(dev, vol), g, step=0.5).cpu().numpy()
+    assert rel_max(got, ref) < 1e-4
+    sino = ctr.forward_project(_t(dev, vol), g, layout=ctr.SINOGRAMS, step=0.5)
+    assert torch.equal(sino.permute(1, 0, 2).cpu(), torch.from_numpy(got))
+
+
+def test_fdk_end_to_end_ball(ctr, orc, dev):
+    """Analytic projections of a uniform ball -> FDK recovers its density."""
+    g = ctr.Geometry(ctr.CONE, nu=128, nv=128, n_views=180, X=64, Y=64, Z=64,
+                     detector_width=256.0, volume_width=64.0, source_distance=200.0,
+                     detector_distance=200.0)
+    ball = np.array([[0, 0, 0, 20, 20, 20, 0, 0, 0, 0.02]], dtype=np.float64)
+    p = orc.analytic_cone(ball, g.source_distance, g.detector_distance, g.detector_width,
+                          g.nu, g.nv, g.n_views).astype(np.float32)
+    nfft = 256
+    staged, _ = ctr.filter_projections(_t(dev, p), g, 0, nfft, ctr.F32, fdk=True)
+    vol = ctr.backproject(staged, g, ctr.F32, scale=g.fbp_scale(nfft)).cpu().numpy()
+    centre = vol[28:36, 28:36, 28:36].mean()
+    assert abs(centre - 0.02) < 0.02 * 0.03
+    assert abs(vol[2:6, 30:34, 2:6].mean()) < 0.02 * 0.05
+
+
+def test_error_paths(ctr, dev):
+    import torch
+    g = _cone(ctr)
+    staged = torch.zeros(16, dtype=torch.uint8, device=dev)
+    with pytest.raises(ctr.CtrError):
+        ctr.backproject(staged, ctr.Geometry(**{**g.__dict__, "source_distance": 1.0}))
+    with pytest.raises(ctr.CtrError):
+        ctr.filter_rows(torch.zeros((2, 100), device=dev), 0, 96)
+    with pytest.raises(ctr.CtrError):
+        ctr.filter_rows(torch.zeros((2, 100), device=dev), 7, 128)
