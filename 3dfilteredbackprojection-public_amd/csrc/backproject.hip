@@ -13,6 +13,9 @@
 //
 // A 256-thread block is 4 wave64s, each covering an 8 x 8 patch of columns,
 // so the 64 lanes of a wave sample a compact detector footprint.
+#include <cstdlib>
+#include <cstring>
+
 #include "ctr_internal.h"
 
 namespace ctr {
@@ -209,11 +212,225 @@ __global__ __launch_bounds__(256) void backproject_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// LDS-staged variant.  Per view, the block's detector footprint -- the
+// bounding box of the projections of its 16 x 16 x TY voxel tile, at most
+// 63 texels wide and HMAX - 1 rows high -- is copied from the staged
+// projections into LDS with global_load_lds (one wave instruction per row,
+// no VGPR round trip), and every tap is then an LDS read: two ds_read2_b32
+// for fp32, one ds_read2_b32 of two 16-bit pairs otherwise.  A view whose
+// footprint does not fit falls back to the direct global taps above.
+// ---------------------------------------------------------------------------
+constexpr int kLdsPitch = 65;  // dwords per LDS row: 64 staged + 1 against bank aliasing
+constexpr int kLdsCols = 64;   // texels staged per row (one per lane)
+
+// Taps at LDS byte address `addr` (row pitch kLdsPitch dwords).
+template <int DT>
+__device__ inline void lds_taps(const unsigned *L, unsigned addr, float2 &top, float2 &bot);
+
+template <>
+__device__ inline void lds_taps<CTR_F32>(const unsigned *L, unsigned addr, float2 &top,
+                                         float2 &bot)
+{
+    const float *F = reinterpret_cast<const float *>(reinterpret_cast<const char *>(L) + addr);
+    top = make_float2(F[0], F[1]);
+    bot = make_float2(F[kLdsPitch], F[kLdsPitch + 1]);
+}
+
+template <>
+__device__ inline void lds_taps<CTR_F16>(const unsigned *L, unsigned addr, float2 &top,
+                                         float2 &bot)
+{
+    const unsigned *P = reinterpret_cast<const unsigned *>(reinterpret_cast<const char *>(L) + addr);
+    const unsigned a = P[0], b = P[kLdsPitch];
+    top = make_float2((float)__builtin_bit_cast(_Float16, (unsigned short)(a & 0xffffu)),
+                      (float)__builtin_bit_cast(_Float16, (unsigned short)(a >> 16)));
+    bot = make_float2((float)__builtin_bit_cast(_Float16, (unsigned short)(b & 0xffffu)),
+                      (float)__builtin_bit_cast(_Float16, (unsigned short)(b >> 16)));
+}
+
+template <>
+__device__ inline void lds_taps<CTR_BF16>(const unsigned *L, unsigned addr, float2 &top,
+                                          float2 &bot)
+{
+    const unsigned *P = reinterpret_cast<const unsigned *>(reinterpret_cast<const char *>(L) + addr);
+    const unsigned a = P[0], b = P[kLdsPitch];
+    top = make_float2(__uint_as_float(a << 16), __uint_as_float(a & 0xffff0000u));
+    bot = make_float2(__uint_as_float(b << 16), __uint_as_float(b & 0xffff0000u));
+}
+
+template <int DT, int TY, int HMAX>
+__global__ __launch_bounds__(256) void backproject_lds_kernel(
+    const void *__restrict__ staged, int pitch_u, int pitch_v, size_t view_texels, int nu,
+    int nv, const float4 *__restrict__ geom, const float *__restrict__ view_scale, int n_views,
+    float *__restrict__ vol, int X, int Y, int Z, float scale, int accumulate, int clip)
+{
+    __shared__ unsigned lds[HMAX * kLdsPitch];
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ti0 = blockIdx.x * 16, tk0 = blockIdx.y * 16;
+    const int i = ti0 + (wave & 1) * 8 + (lane & 7);
+    const int k = tk0 + (wave >> 1) * 8 + (lane >> 3);
+    const int j0 = blockIdx.z * TY;
+    const float fi = (float)i, fk = (float)k, fj0 = (float)j0;
+    const float fnu = (float)nu, fnv = (float)nv;
+    // tile corners (voxel-centre indices) for the footprint
+    const float ci[2] = {(float)ti0, (float)(ti0 + 15)};
+    const float ck[2] = {(float)tk0, (float)(tk0 + 15)};
+    const float cj[2] = {fj0, (float)(j0 + TY - 1)};
+
+    float acc[TY];
+#pragma unroll
+    for (int t = 0; t < TY; ++t) acc[t] = 0.0f;
+
+    const char *vbase = reinterpret_cast<const char *>(staged);
+    const size_t view_bytes = view_texels * 4;
+    const unsigned pitch4 = (unsigned)pitch_u * 4u;
+    const float fpitch4 = (float)pitch4;
+    const float fcol0 = (float)(pitch4 + 4u);
+
+#pragma unroll 1
+    for (int n = 0; n < n_views; ++n) {
+        const float4 ga = geom[3 * n + 0];
+        const float4 gb = geom[3 * n + 1];
+        const float4 gc = geom[3 * n + 2];
+        float vs = scale;
+        if (view_scale) vs *= view_scale[n];
+
+        // footprint of the tile: every lane evaluates the 8 corners with the
+        // same arithmetic as the voxels below, so the bounds are uniform
+        float umin = 3.0e38f, umax = -3.0e38f, vmin = 3.0e38f, vmax = -3.0e38f;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const float cden = fmaf(gc.x, ci[a], fmaf(gc.y, ck[b], gc.z));
+                const float cinv = __builtin_amdgcn_rcpf(cden);
+                const float cu = fmaf(ga.x, ci[a], fmaf(ga.y, ck[b], ga.z)) * cinv;
+                const float cvb = fmaf(gb.x, ci[a], fmaf(gb.y, ck[b], gb.z)) * cinv;
+                const float cdv = ga.w * cinv;
+                umin = fminf(umin, cu);
+                umax = fmaxf(umax, cu);
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    const float cv = fmaf(cdv, cj[c], cvb);
+                    vmin = fminf(vmin, cv);
+                    vmax = fmaxf(vmax, cv);
+                }
+            }
+        umin = __builtin_amdgcn_fmed3f(umin, -1.0f, fnu);
+        umax = __builtin_amdgcn_fmed3f(umax, -1.0f, fnu);
+        vmin = __builtin_amdgcn_fmed3f(vmin, -1.0f, fnv);
+        vmax = __builtin_amdgcn_fmed3f(vmax, -1.0f, fnv);
+        // detector texel ranges (inclusive) holding every tap, 0.01 px margin
+        const int u0 = __builtin_amdgcn_readfirstlane(max((int)floorf(umin - 0.01f), -1));
+        const int u1 = __builtin_amdgcn_readfirstlane(min((int)floorf(umax + 0.01f) + 1, nu + 1));
+        const int v0 = __builtin_amdgcn_readfirstlane(max((int)floorf(vmin - 0.01f), -1));
+        const int v1 = __builtin_amdgcn_readfirstlane(min((int)floorf(vmax + 0.01f) + 1, nv + 1));
+        const int W = u1 - u0 + 1, H = v1 - v0 + 1;
+
+        const float den = fmaf(gc.x, fi, fmaf(gc.y, fk, gc.z));
+        const float inv = __builtin_amdgcn_rcpf(den);
+        float fu = fmaf(ga.x, fi, fmaf(ga.y, fk, ga.z)) * inv;
+        const float fvb = fmaf(gb.x, fi, fmaf(gb.y, fk, gb.z)) * inv;
+        const float dv = ga.w * inv;
+        const float w2 = inv * inv * vs;
+        fu = __builtin_amdgcn_fmed3f(fu, -1.0f, fnu);
+        const char *vptr = vbase + (size_t)n * view_bytes;
+
+        if (W < kLdsCols && H < HMAX) {
+            // stage rows v0 .. v1 + 1 (one spare row, clamped into the view)
+            const unsigned *src = reinterpret_cast<const unsigned *>(vptr);
+            const int scol = min(u0 + 1 + lane, pitch_u - 1);
+            for (int r = wave; r <= H; r += 4) {
+                const int srow = min(v0 + 1 + r, pitch_v - 1);
+                __builtin_amdgcn_global_load_lds(
+                    (const void __attribute__((address_space(1))) *)(src + (size_t)srow * pitch_u + scol),
+                    (void __attribute__((address_space(3))) *)(lds + r * kLdsPitch), 4, 0, 0);
+            }
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();
+
+            const float ful = __builtin_amdgcn_fmed3f(fu - (float)u0, 0.0f, (float)(W - 1));
+            const float fu0 = floorf(ful);
+            const float wu = ful - fu0;
+            const unsigned colb = (unsigned)fu0 * 4u;
+            const float fvl = fmaf(dv, fj0, fvb) - (float)v0;
+            const float vhi = (float)(H - 1);
+#pragma unroll
+            for (int t = 0; t < TY; ++t) {
+                const float fv = __builtin_amdgcn_fmed3f(fmaf(dv, (float)t, fvl), 0.0f, vhi);
+                const float fv0 = floorf(fv);
+                const float wv = fv - fv0;
+                const unsigned addr = __umul24((unsigned)fv0, kLdsPitch * 4u) + colb;
+                float2 top, bot;
+                lds_taps<DT>(lds, addr, top, bot);
+                const float ht = fmaf(wu, top.y - top.x, top.x);
+                const float hb = fmaf(wu, bot.y - bot.x, bot.x);
+                acc[t] = fmaf(w2, fmaf(wv, hb - ht, ht), acc[t]);
+            }
+            __syncthreads();
+        } else {
+            const float fu0 = floorf(fu);
+            const float wu = fu - fu0;
+            const float fbase = fmaf(fu0, 4.0f, fcol0);
+            const float fv_j0 = fmaf(dv, fj0, fvb);
+#pragma unroll
+            for (int t = 0; t < TY; ++t) {
+                const float fv = __builtin_amdgcn_fmed3f(fmaf(dv, (float)t, fv_j0), -1.0f, fnv);
+                const float fv0 = floorf(fv);
+                const float wv = fv - fv0;
+                const unsigned off = (unsigned)fmaf(fv0, fpitch4, fbase);
+                const float2 top = Taps<DT>::load(vptr, off);
+                const float2 bot = Taps<DT>::load(vptr, off + pitch4);
+                const float ht = fmaf(wu, top.y - top.x, top.x);
+                const float hb = fmaf(wu, bot.y - bot.x, bot.x);
+                acc[t] = fmaf(w2, fmaf(wv, hb - ht, ht), acc[t]);
+            }
+        }
+    }
+
+    if (i >= X || k >= Z) return;
+#pragma unroll
+    for (int t = 0; t < TY; ++t) {
+        const int j = j0 + t;
+        if (j < Y) {
+            const size_t idx = ((size_t)k * Y + j) * X + i;
+            float v = acc[t];
+            if (accumulate) v += vol[idx];
+            if (clip) v = fmaxf(v, 0.0f);
+            vol[idx] = v;
+        }
+    }
+}
+
+// Kernel choice: the LDS-staged kernel for volumes at least 32 voxels along
+// the rotation axis, the direct-tap kernel otherwise.  CTR_BACKPROJECT_KERNEL
+// = "gather" or "lds" overrides (diagnostics and A/B timing).
+static int variant_override()
+{
+    const char *e = getenv("CTR_BACKPROJECT_KERNEL");
+    if (!e) return 0;
+    if (!strcmp(e, "gather")) return 1;
+    if (!strcmp(e, "lds")) return 2;
+    return 0;
+}
+
 template <int DT>
 static int launch_bp(hipStream_t s, int vy, const void *staged, const StagedLayout &L, int nu,
                      int nv, const float4 *geom, const float *view_scale, int n_views,
                      float *vol, int X, int Y, int Z, float scale, int accumulate, int clip)
 {
+    const int ov = variant_override();
+    if ((ov == 2 || (ov == 0 && Y >= 32))) {
+        constexpr int TY = 32, HMAX = 112;
+        dim3 grid((X + 15) / 16, (Z + 15) / 16, (Y + TY - 1) / TY);
+        backproject_lds_kernel<DT, TY, HMAX><<<grid, 256, 0, s>>>(
+            staged, L.pitch_u, L.pitch_v, L.view, nu, nv, geom, view_scale, n_views, vol, X, Y,
+            Z, scale, accumulate, clip);
+        CTR_CHECK_LAUNCH();
+        return CTR_OK;
+    }
     dim3 block(256);
 #define CTR_BP_CASE(V)                                                                   \
     case V: {                                                                            \

@@ -67,6 +67,8 @@ def lib():
                                        _i, _d, _d, _d, _p, _i]
         L.orc_forward_parallel.argtypes = [_p, _i, _i, _i, _d, _d, _i, _i, _i,
                                            _d, _d, _d, _p, _i]
+        L.orc_rebin_fan_to_parallel.argtypes = [_p, _i, _i, _d, _d, _d, _d, _p,
+                                                _i, _d]
         L.orc_ellipsoid_line_integral.argtypes = [_p, _i, _p, _p]
         L.orc_ellipsoid_line_integral.restype = _d
         L.orc_analytic_cone.argtypes = [_p, _i, _d, _d, _d, _i, _i, _i, _d, _d,
@@ -299,6 +301,18 @@ def forward_parallel(vol, volume_width, detector_width, nu, nv, n_views,
                                offset, step, _ptr(proj),
                                threads or threads_default())
     return proj
+
+
+def rebin_fan_to_parallel(fan, sod, odd, detector_width, rotation, nu_par,
+                          parallel_width):
+    """fan sinogram (N, nu_fan) -> parallel sinogram (N, nu_par), FP64."""
+    fan = _f32(fan)
+    n, nu_fan = fan.shape
+    out = np.zeros((n, nu_par), dtype=np.float64)
+    lib().orc_rebin_fan_to_parallel(_ptr(fan), nu_fan, n, sod, odd,
+                                    detector_width, rotation, _ptr(out), nu_par,
+                                    parallel_width)
+    return out
 
 
 # --- analytic phantoms -----------------------------------------------------

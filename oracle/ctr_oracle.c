@@ -539,6 +539,51 @@ void orc_forward_parallel(const float *vol, int X, int Y, int Z,
         }
 }
 
+void orc_rebin_fan_to_parallel(const float *fan, int nu_fan, int n_views,
+                               double sod, double odd, double detector_width,
+                               double rotation_count, double *par, int nu_par,
+                               double parallel_width)
+{
+    /* Parallel ray (theta, s) is the fan ray of source angle
+     * beta = theta + gamma at detector u = SDD tan(gamma), sin(gamma) =
+     * s / SOD (flat detector at SDD, same angle convention as the
+     * projectors).  Bilinear in (view, u); periodic in views for one full
+     * turn, zero outside otherwise. */
+    double sdd = sod + odd, delta = detector_width / nu_fan;
+    double dth = 2.0 * M_PI * rotation_count / n_views;
+    double pitch = parallel_width / nu_par;
+    int periodic = fabs(rotation_count - 1.0) < 1e-12;
+    for (int n = 0; n < n_views; n++)
+        for (int is = 0; is < nu_par; is++) {
+            double s = (is + 0.5 - 0.5 * nu_par) * pitch, out = 0.0;
+            if (fabs(s) < sod) {
+                double gam = asin(s / sod);
+                double fb = (dth * n + gam) / dth;
+                double fu = sdd * tan(gam) / delta + 0.5 * nu_fan - 0.5;
+                if (fu > -1.0 && fu < nu_fan) {
+                    double fb0 = floor(fb), fu0 = floor(fu);
+                    for (int b = 0; b < 2; b++) {
+                        long vb = (long)fb0 + b;
+                        if (periodic) {
+                            vb %= n_views;
+                            if (vb < 0) vb += n_views;
+                        } else if (vb < 0 || vb >= n_views) {
+                            continue;
+                        }
+                        double wb = b ? fb - fb0 : 1.0 - (fb - fb0);
+                        for (int a = 0; a < 2; a++) {
+                            int c = (int)fu0 + a;
+                            if (c < 0 || c >= nu_fan) continue;
+                            double wa = a ? fu - fu0 : 1.0 - (fu - fu0);
+                            out += wb * wa * (double)fan[(size_t)vb * nu_fan + c];
+                        }
+                    }
+                }
+            }
+            par[(size_t)n * nu_par + is] = out;
+        }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Analytic phantoms                                                         */
 /* ------------------------------------------------------------------------ */

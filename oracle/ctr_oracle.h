@@ -135,6 +135,16 @@ void orc_forward_parallel(const float *vol, int X, int Y, int Z,
                           double rotation_offset, double step, double *proj,
                           int threads);
 
+/* ---- Fan -> parallel rebinning (FBP2D::parallelize_fan_beam_sinogram,
+ * Source/FilteredBackProjection/FBP2D.cpp:31-52; the reference's GLSL is
+ * empty, the resampling rule is stated in ctr_oracle.c).
+ * fan (N, nu_fan) over 2*pi*rotation_count -> par (N, nu_par) with pitch
+ * parallel_width / nu_par, same view angles. */
+void orc_rebin_fan_to_parallel(const float *fan, int nu_fan, int n_views,
+                               double sod, double odd, double detector_width,
+                               double rotation_count, double *par, int nu_par,
+                               double parallel_width);
+
 /* ---- Analytic phantoms ----------------------------------------------------
  * Ellipsoid record (10 doubles): cx cy cz  a b c  alpha beta gamma  density,
  * world units.  Body-to-world rotation R = Rz(gamma) Ry(beta) Rx(alpha).
