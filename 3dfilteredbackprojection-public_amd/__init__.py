@@ -86,6 +86,21 @@ SIGNATURES = {
 _RESTYPE = {"ctr_last_error": ctypes.c_char_p}
 
 
+class BackprojectJob(ctypes.Structure):
+    """ctr_backproject_job of include/ctrecon.h."""
+    _fields_ = [("staged", _v), ("view_scale", _v), ("volume", _v),
+                ("geometry", _i), ("nu", _i), ("nv", _i), ("n_views", _i),
+                ("X", _i), ("Y", _i), ("Z", _i),
+                ("source_distance", _d), ("detector_distance", _d),
+                ("detector_width", _d), ("volume_width", _d),
+                ("rotation_count", _d), ("rotation_offset", _d),
+                ("scale", _f), ("accumulate", _i), ("clip_negative", _i)]
+
+
+SIGNATURES["ctr_backproject_packed"] = [_v, _v, _i, ctypes.POINTER(BackprojectJob), _i]
+MAX_PACKED = 16
+
+
 def lib():
     """The loaded libctrecon.so; raises if the HIP library is not built."""
     global _LIB
@@ -314,6 +329,25 @@ def backproject(staged, geom, dtype=F32, view_scale=None, scale=1.0, volume=None
         geom.X, geom.Y, geom.Z, float(scale), 1 if accumulate else 0,
         1 if clip else 0))
     return volume
+
+
+def backproject_packed(items, dtype=F32, ws=None, stream=None):
+    """One packed launch over up to 16 reconstructions.
+
+    items: list of (staged, geom, view_scale, scale, volume, clip); returns
+    the volumes."""
+    if not 1 <= len(items) <= MAX_PACKED:
+        raise ValueError(f"pack 1..{MAX_PACKED} jobs, got {len(items)}")
+    jobs = (BackprojectJob * len(items))()
+    for q, (staged, g, vs, scale, vol, clip) in enumerate(items):
+        jobs[q] = BackprojectJob(
+            staged.data_ptr(), vs.data_ptr() if vs is not None else None, vol.data_ptr(),
+            g.kind, g.nu, g.nv, g.n_views, g.X, g.Y, g.Z, g.source_distance,
+            g.detector_distance, g.detector_width, g.volume_width, g.rotation_count,
+            g.rotation_offset, float(scale), 0, 1 if clip else 0)
+    check(lib().ctr_backproject_packed(stream_handle(stream), ws.handle if ws else None,
+                                       dtype, jobs, len(items)))
+    return [it[4] for it in items]
 
 
 def forward_project(volume, geom, layout=PROJECTIONS, step=1.0, out=None, stream=None):

@@ -1,15 +1,21 @@
 // backproject.hip -- voxel-driven cone / parallel backprojection for gfx950.
 //
 // One launch covers every view.  A thread owns one (x, z) column and a run of
-// VY consecutive voxels along the rotation axis y, accumulating them in
+// consecutive voxels along the rotation axis y, accumulating them in
 // registers across all views; the volume is written once at the end.  For a
 // fixed column the detector u coordinate, the distance weight and the v step
-// are the same for the whole run, so per view a thread spends one reciprocal
-// and a handful of FMAs on the column and then, per voxel, one FMA for fv,
-// a clamp, two 8-byte tap loads (fp32) or two 4-byte pair loads (fp16/bf16)
-// and the bilinear blend on the VALU.  Taps read the zero-bordered staged
-// layout, so clamping fu to [-1, nu] and fv to [-1, nv] implements "zero
-// outside the detector" without branches.
+// are the same for the whole run, so per view a thread sets the column up
+// once -- in FP64, from FP64 per-view projection matrices -- and then spends
+// 11 FP32 VALU ops per voxel: the v FMA, a clamp, fract + cvt for the row
+// and its weight, a 24-bit mad for the texel address, two vertical lerps and
+// two pre-weighted accumulations.
+//
+// Coordinates are "staged coordinates" (one border texel before the data),
+// where clamping to [0, n + 1] is the same as "zero outside the detector"
+// and truncation equals floor.  The v coordinate of a run is split into an
+// integer row I and a fraction f < 1 in FP64, so the per-voxel FP32 value
+// f + dv * t stays small and keeps ~1e-5 px resolution even on 1536-row
+// panels (a plain FP32 row coordinate has 1.2e-4 px steps there).
 //
 // A 256-thread block is 4 wave64s, each covering an 8 x 8 patch of columns,
 // so the 64 lanes of a wave sample a compact detector footprint.
@@ -20,9 +26,12 @@
 
 namespace ctr {
 
-// Per-view projective rows, FP64 -> FP32 (see build_geometry).
-__global__ void geometry_kernel(float4 *rec, int n_views, int geometry, double rot,
-                                double offset, double sod, double sdd, double delta,
+constexpr int kGeom64 = 12;  // doubles per view: Ui Uk U0 Vj Vi Vk V0 Di Dk D0 - -
+
+// Per-view projective rows (see build_geometry): FP32 copies for the tile
+// footprints and FP64 rows for the voxel coordinates.
+__global__ void geometry_kernel(float4 *rec, double *rec64, int n_views, int geometry,
+                                double rot, double offset, double sod, double sdd, double delta,
                                 double d, int X, int Y, int Z, int nu, int nv)
 {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -61,6 +70,19 @@ __global__ void geometry_kernel(float4 *rec, int n_views, int geometry, double r
     rec[3 * n + 0] = make_float4((float)Ui, (float)Uk, (float)U0, (float)Vj);
     rec[3 * n + 1] = make_float4((float)Vi, (float)Vk, (float)V0, 0.0f);
     rec[3 * n + 2] = make_float4((float)Di, (float)Dk, (float)D0, 0.0f);
+    double *r = rec64 + (size_t)kGeom64 * n;
+    r[0] = Ui;
+    r[1] = Uk;
+    r[2] = U0;
+    r[3] = Vj;
+    r[4] = Vi;
+    r[5] = Vk;
+    r[6] = V0;
+    r[7] = Di;
+    r[8] = Dk;
+    r[9] = D0;
+    r[10] = 0.0;
+    r[11] = 0.0;
 }
 
 int check_geometry(int geometry, double sod, double odd, double detector_width,
@@ -72,6 +94,9 @@ int check_geometry(int geometry, double sod, double odd, double detector_width,
                 CTR_ERR_INVALID_ARGUMENT,
                 "sizes must be positive: volume (%d, %d, %d), detector (%d, %d), views %d",
                 X, Y, Z, nu, nv, n_views);
+    CTR_REQUIRE(X < (1 << 20) && Y < (1 << 20) && Z < (1 << 20) && nu < (1 << 20) &&
+                    nv < (1 << 20),
+                CTR_ERR_INVALID_ARGUMENT, "sizes must be below 2^20");
     CTR_REQUIRE(detector_width > 0 && volume_width > 0, CTR_ERR_INVALID_ARGUMENT,
                 "detector_width and volume_width must be positive");
     if (geometry == CTR_GEOMETRY_CONE) {
@@ -85,19 +110,46 @@ int check_geometry(int geometry, double sod, double odd, double detector_width,
     return CTR_OK;
 }
 
+// Bytes of geometry records per view (FP32 footprint rows + FP64 rows).
+constexpr size_t kGeomBytesPerView = kGeometryBytesPerView;
+static_assert(kGeometryBytesPerView == sizeof(float4) * 3 + sizeof(double) * kGeom64, "layout");
+
+struct GeomPtrs {
+    const float4 *f32;
+    const double *f64;
+};
+
+// Writes the records of `n_views` views at `base` (room for `n_views`
+// float4 triples followed by the FP64 rows).
+static int build_geometry_at(hipStream_t s, void *base, int geometry, int n_views, double sod,
+                             double odd, double detector_width, double volume_width,
+                             double rotation_count, double rotation_offset, int X, int Y, int Z,
+                             int nu, int nv, GeomPtrs *out)
+{
+    float4 *f = (float4 *)base;
+    double *g = (double *)(f + 3 * (size_t)n_views);
+    geometry_kernel<<<(n_views + 63) / 64, 64, 0, s>>>(
+        f, g, n_views, geometry, rotation_count, rotation_offset, sod, sod + odd,
+        detector_width / nu, volume_width / X, X, Y, Z, nu, nv);
+    CTR_CHECK_LAUNCH();
+    out->f32 = f;
+    out->f64 = g;
+    return CTR_OK;
+}
+
 int build_geometry(hipStream_t s, ctr_workspace *ws, int geometry, int n_views, double sod,
                    double odd, double detector_width, double volume_width,
                    double rotation_count, double rotation_offset, int X, int Y, int Z, int nu,
                    int nv, const float4 **out)
 {
     void *p = nullptr;
-    int rc = workspace_get(ws, SLOT_GEOMETRY, sizeof(float4) * 3 * (size_t)n_views, &p);
+    int rc = workspace_get(ws, SLOT_GEOMETRY, kGeomBytesPerView * (size_t)n_views, &p);
     if (rc) return rc;
-    geometry_kernel<<<(n_views + 63) / 64, 64, 0, s>>>(
-        (float4 *)p, n_views, geometry, rotation_count, rotation_offset, sod, sod + odd,
-        detector_width / nu, volume_width / X, X, Y, Z, nu, nv);
-    CTR_CHECK_LAUNCH();
-    *out = (const float4 *)p;
+    GeomPtrs g;
+    if ((rc = build_geometry_at(s, p, geometry, n_views, sod, odd, detector_width, volume_width,
+                                rotation_count, rotation_offset, X, Y, Z, nu, nv, &g)))
+        return rc;
+    *out = g.f32;
     return CTR_OK;
 }
 
@@ -117,14 +169,22 @@ struct Taps<CTR_F32> {
     }
 };
 
+__device__ inline float2 f16_pair(unsigned w)
+{
+    return make_float2((float)__builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu)),
+                       (float)__builtin_bit_cast(_Float16, (unsigned short)(w >> 16)));
+}
+
+__device__ inline float2 bf16_pair(unsigned w)
+{
+    return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
+}
+
 template <>
 struct Taps<CTR_F16> {
     __device__ static inline float2 load(const char *base, unsigned off)
     {
-        unsigned w = *reinterpret_cast<const unsigned *>(base + off);
-        _Float16 lo = __builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu));
-        _Float16 hi = __builtin_bit_cast(_Float16, (unsigned short)(w >> 16));
-        return make_float2((float)lo, (float)hi);
+        return f16_pair(*reinterpret_cast<const unsigned *>(base + off));
     }
 };
 
@@ -132,72 +192,88 @@ template <>
 struct Taps<CTR_BF16> {
     __device__ static inline float2 load(const char *base, unsigned off)
     {
-        unsigned w = *reinterpret_cast<const unsigned *>(base + off);
-        return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
+        return bf16_pair(*reinterpret_cast<const unsigned *>(base + off));
     }
 };
 
-template <int DT, int VY>
-__global__ __launch_bounds__(256) void backproject_kernel(
-    const void *__restrict__ staged, int pitch_u, size_t view_texels, float fnu, float fnv,
-    const float4 *__restrict__ geom, const float *__restrict__ view_scale, int n_views,
-    float *__restrict__ vol, int X, int Y, int Z, float scale, int accumulate, int clip)
+// Column quantities of one view, FP64 from the FP64 projection rows.
+struct ColumnView {
+    double su;   // staged u coordinate
+    double sv0;  // staged v coordinate of the run's first voxel
+    float dv;    // v step per voxel along y
+    float w2;    // distance weight squared times the scale
+};
+
+__device__ inline ColumnView column_view64(const double *g, double fi, double fk, double fj0,
+                                           float vs)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int i = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
-    const int k = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
-    const int j0 = blockIdx.z * VY;
-    const float fi = (float)i, fk = (float)k;
+    ColumnView c;
+    const double den = fma(g[7], fi, fma(g[8], fk, g[9]));
+    // reciprocal with one Newton step: full FP64 precision
+    double r = __builtin_amdgcn_rcp(den);
+    r = fma(r, fma(-den, r, 1.0), r);
+    c.su = fma(g[0], fi, fma(g[1], fk, g[2])) * r + 1.0;
+    c.sv0 = fma(g[3], fj0, fma(g[4], fi, fma(g[5], fk, g[6]))) * r + 1.0;
+    c.dv = (float)(g[3] * r);
+    c.w2 = (float)(r * r) * vs;
+    return c;
+}
 
-    float acc[VY];
+// Per-voxel constants of one column run: taps at byte offset
+// mad24(row, pitch, base) of the staged view (or LDS footprint), row =
+// (unsigned)x with x = clamp(f + dv * t, lo, hi) >= 0.
+struct Run {
+    float cL, cR;   // (1 - wu) w2, wu w2
+    float f, dv;    // fractional row of the first voxel, row step
+    float lo, hi;   // clamp of x (rows relative to I)
+    unsigned base;  // byte offset of row I, column iu
+};
+
+// su_range / rows: clamp interval of su and of the absolute row; org_u /
+// org_v: texel coordinates of byte offset 0; pitch in bytes.
+__device__ inline Run make_run(const ColumnView &c, double su_hi, double org_u, int row_lo,
+                               int row_hi, int org_v, unsigned pitch, double su_global_hi)
+{
+    Run R;
+    const double sug = fmin(fmax(c.su, 0.0), su_global_hi);
+    const double su = fmin(fmax(sug - org_u, 0.0), su_hi);
+    const double sfl = floor(su);
+    const float wu = (float)(su - sfl);
+    R.cR = c.w2 * wu;
+    R.cL = c.w2 - R.cR;
+    // rows: keep the run start inside [row_lo - 2^20, row_hi]
+    const double sv0 = fmin(fmax(c.sv0, (double)(row_lo - (1 << 20))), (double)row_hi);
+    const double ifl = floor(sv0);
+    const int I = (int)ifl;
+    R.f = (float)(sv0 - ifl);
+    R.dv = c.dv;
+    R.lo = (float)(row_lo - I);
+    R.hi = (float)(row_hi - I);
+    R.base = (unsigned)sfl * 4u + (unsigned)(I - org_v) * pitch;
+    return R;
+}
+
+template <int DT, int VY>
+__device__ inline void gather_run(float (&acc)[VY], const Run &R, const char *top,
+                                  const char *bot, unsigned pitch)
+{
 #pragma unroll
-    for (int t = 0; t < VY; ++t) acc[t] = 0.0f;
-
-    const char *vbase = reinterpret_cast<const char *>(staged);
-    const size_t view_bytes = view_texels * 4;
-    const unsigned pitch4 = (unsigned)pitch_u * 4u;
-    const float fpitch4 = (float)pitch4;
-    const float fcol0 = (float)(pitch4 + 4u);  // +1 row, +1 column of border
-    const float fj0 = (float)j0;
-
-#pragma unroll 1
-    for (int n = 0; n < n_views; ++n) {
-        const float4 ga = geom[3 * n + 0];
-        const float4 gb = geom[3 * n + 1];
-        const float4 gc = geom[3 * n + 2];
-        float vs = scale;
-        if (view_scale) vs *= view_scale[n];
-
-        const float den = fmaf(gc.x, fi, fmaf(gc.y, fk, gc.z));
-        const float inv = __builtin_amdgcn_rcpf(den);
-        float fu = fmaf(ga.x, fi, fmaf(ga.y, fk, ga.z)) * inv;
-        const float fvb = fmaf(gb.x, fi, fmaf(gb.y, fk, gb.z)) * inv;
-        const float dv = ga.w * inv;
-        const float w2 = inv * inv * vs;
-
-        fu = __builtin_amdgcn_fmed3f(fu, -1.0f, fnu);
-        const float fu0 = floorf(fu);
-        const float wu = fu - fu0;
-        // byte offset of staged texel (col, row) = ((fv0 + 1) * pitch + fu0 + 1) * 4;
-        // every term is an integer below 2^24, so the float arithmetic is exact
-        const float fbase = fmaf(fu0, 4.0f, fcol0);
-        const float fv_j0 = fmaf(dv, fj0, fvb);
-        const char *vptr = vbase + (size_t)n * view_bytes;
-
-#pragma unroll
-        for (int t = 0; t < VY; ++t) {
-            const float fv = __builtin_amdgcn_fmed3f(fmaf(dv, (float)t, fv_j0), -1.0f, fnv);
-            const float fv0 = floorf(fv);
-            const float wv = fv - fv0;
-            const unsigned off = (unsigned)fmaf(fv0, fpitch4, fbase);
-            const float2 top = Taps<DT>::load(vptr, off);
-            const float2 bot = Taps<DT>::load(vptr, off + pitch4);
-            const float ht = fmaf(wu, top.y - top.x, top.x);
-            const float hb = fmaf(wu, bot.y - bot.x, bot.x);
-            acc[t] = fmaf(w2, fmaf(wv, hb - ht, ht), acc[t]);
-        }
+    for (int t = 0; t < VY; ++t) {
+        const float x = __builtin_amdgcn_fmed3f(fmaf(R.dv, (float)t, R.f), R.lo, R.hi);
+        const float wv = __builtin_amdgcn_fractf(x);
+        const unsigned off = __umul24((unsigned)x, pitch) + R.base;
+        const float2 a = Taps<DT>::load(top, off);
+        const float2 b = Taps<DT>::load(bot, off);
+        const float left = fmaf(wv, b.x - a.x, a.x);
+        const float right = fmaf(wv, b.y - a.y, a.y);
+        acc[t] = fmaf(R.cR, right, fmaf(R.cL, left, acc[t]));
     }
+}
 
+template <int VY>
+__device__ inline void store_run(const float (&acc)[VY], float *vol, int i, int j0, int k, int X,
+                                 int Y, int Z, int accumulate, int clip)
+{
     if (i >= X || k >= Z) return;
 #pragma unroll
     for (int t = 0; t < VY; ++t) {
@@ -212,19 +288,76 @@ __global__ __launch_bounds__(256) void backproject_kernel(
     }
 }
 
+// One reconstruction's backprojection arguments; a packed launch carries
+// several of them as a kernel argument (no device-side table to upload).
+struct BpJob {
+    const void *staged;
+    const float4 *geom;
+    const double *geom64;
+    const float *view_scale;
+    float *vol;
+    size_t view_texels;
+    int pitch_u, pitch_v, nu, nv, n_views;
+    int X, Y, Z;
+    float scale;
+    int accumulate, clip;
+    int gx, gy;       // tiles along x and z
+    int block_begin;  // first flat block of this job in a packed launch
+};
+
+constexpr int kMaxPacked = 16;
+struct BpBatch {
+    BpJob job[kMaxPacked];
+    int count;
+};
+
 // ---------------------------------------------------------------------------
-// LDS-staged variant.  Per view, the block's detector footprint -- the
+// Direct-tap kernel: taps straight from the staged buffer (global loads
+// through L1/L2).
+// ---------------------------------------------------------------------------
+template <int DT, int VY>
+__global__ __launch_bounds__(256) void backproject_kernel(BpJob J)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    const int k = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const int j0 = blockIdx.z * VY;
+
+    float acc[VY];
+#pragma unroll
+    for (int t = 0; t < VY; ++t) acc[t] = 0.0f;
+
+    const char *vbase = reinterpret_cast<const char *>(J.staged);
+    const size_t view_bytes = J.view_texels * 4;
+    const unsigned pitch4 = (unsigned)J.pitch_u * 4u;
+    const double su_hi = (double)J.nu + 1.0;
+
+#pragma unroll 1
+    for (int n = 0; n < J.n_views; ++n) {
+        float vs = J.scale;
+        if (J.view_scale) vs *= J.view_scale[n];
+        const ColumnView c =
+            column_view64(J.geom64 + (size_t)kGeom64 * n, i, k, j0, vs);
+        const Run R = make_run(c, su_hi, 0.0, 0, J.nv + 1, 0, pitch4, su_hi);
+        const char *top = vbase + (size_t)n * view_bytes;
+        gather_run<DT, VY>(acc, R, top, top + pitch4, pitch4);
+    }
+    store_run<VY>(acc, J.vol, i, j0, k, J.X, J.Y, J.Z, J.accumulate, J.clip);
+}
+
+// ---------------------------------------------------------------------------
+// LDS-staged kernel.  Per view, the block's detector footprint -- the
 // bounding box of the projections of its 16 x 16 x TY voxel tile, at most
 // 63 texels wide and HMAX - 1 rows high -- is copied from the staged
 // projections into LDS with global_load_lds (one wave instruction per row,
 // no VGPR round trip), and every tap is then an LDS read: two ds_read2_b32
 // for fp32, one ds_read2_b32 of two 16-bit pairs otherwise.  A view whose
-// footprint does not fit falls back to the direct global taps above.
+// footprint does not fit falls back to the direct global taps.
 // ---------------------------------------------------------------------------
 constexpr int kLdsPitch = 65;  // dwords per LDS row: 64 staged + 1 against bank aliasing
 constexpr int kLdsCols = 64;   // texels staged per row (one per lane)
 
-// Taps at LDS byte address `addr` (row pitch kLdsPitch dwords).
+// Taps at LDS byte address `addr`.
 template <int DT>
 __device__ inline void lds_taps(const unsigned *L, unsigned addr, float2 &top, float2 &bot);
 
@@ -242,11 +375,8 @@ __device__ inline void lds_taps<CTR_F16>(const unsigned *L, unsigned addr, float
                                          float2 &bot)
 {
     const unsigned *P = reinterpret_cast<const unsigned *>(reinterpret_cast<const char *>(L) + addr);
-    const unsigned a = P[0], b = P[kLdsPitch];
-    top = make_float2((float)__builtin_bit_cast(_Float16, (unsigned short)(a & 0xffffu)),
-                      (float)__builtin_bit_cast(_Float16, (unsigned short)(a >> 16)));
-    bot = make_float2((float)__builtin_bit_cast(_Float16, (unsigned short)(b & 0xffffu)),
-                      (float)__builtin_bit_cast(_Float16, (unsigned short)(b >> 16)));
+    top = f16_pair(P[0]);
+    bot = f16_pair(P[kLdsPitch]);
 }
 
 template <>
@@ -254,96 +384,91 @@ __device__ inline void lds_taps<CTR_BF16>(const unsigned *L, unsigned addr, floa
                                           float2 &bot)
 {
     const unsigned *P = reinterpret_cast<const unsigned *>(reinterpret_cast<const char *>(L) + addr);
-    const unsigned a = P[0], b = P[kLdsPitch];
-    top = make_float2(__uint_as_float(a << 16), __uint_as_float(a & 0xffff0000u));
-    bot = make_float2(__uint_as_float(b << 16), __uint_as_float(b & 0xffff0000u));
+    top = bf16_pair(P[0]);
+    bot = bf16_pair(P[kLdsPitch]);
+}
+
+// FP32 staged u and v of a voxel from the FP32 rows (footprints only).
+__device__ inline void corner_uv(const float4 &ga, const float4 &gb, const float4 &gc, float fi,
+                                 float fk, float fj_a, float fj_b, float &su, float &sva,
+                                 float &svb)
+{
+    const float den = fmaf(gc.x, fi, fmaf(gc.y, fk, gc.z));
+    const float inv = __builtin_amdgcn_rcpf(den);
+    su = fmaf(ga.x, fi, fmaf(ga.y, fk, ga.z)) * inv + 1.0f;
+    const float fvb = fmaf(gb.x, fi, fmaf(gb.y, fk, gb.z)) * inv + 1.0f;
+    const float dv = ga.w * inv;
+    sva = fmaf(dv, fj_a, fvb);
+    svb = fmaf(dv, fj_b, fvb);
 }
 
 template <int DT, int TY, int HMAX>
-__global__ __launch_bounds__(256) void backproject_lds_kernel(
-    const void *__restrict__ staged, int pitch_u, int pitch_v, size_t view_texels, int nu,
-    int nv, const float4 *__restrict__ geom, const float *__restrict__ view_scale, int n_views,
-    float *__restrict__ vol, int X, int Y, int Z, float scale, int accumulate, int clip)
+__device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int bz,
+                                            unsigned *lds)
 {
-    __shared__ unsigned lds[HMAX * kLdsPitch];
-
+    const int pitch_u = J.pitch_u, pitch_v = J.pitch_v, nu = J.nu, nv = J.nv;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int ti0 = blockIdx.x * 16, tk0 = blockIdx.y * 16;
+    const int ti0 = bx * 16, tk0 = by * 16;
     const int i = ti0 + (wave & 1) * 8 + (lane & 7);
     const int k = tk0 + (wave >> 1) * 8 + (lane >> 3);
-    const int j0 = blockIdx.z * TY;
-    const float fi = (float)i, fk = (float)k, fj0 = (float)j0;
+    const int j0 = bz * TY;
     const float fnu = (float)nu, fnv = (float)nv;
+    const double su_hi = (double)nu + 1.0;
     // tile corners (voxel-centre indices) for the footprint
     const float ci[2] = {(float)ti0, (float)(ti0 + 15)};
     const float ck[2] = {(float)tk0, (float)(tk0 + 15)};
-    const float cj[2] = {fj0, (float)(j0 + TY - 1)};
+    const float fj_a = (float)j0, fj_b = (float)(j0 + TY - 1);
 
     float acc[TY];
 #pragma unroll
     for (int t = 0; t < TY; ++t) acc[t] = 0.0f;
 
-    const char *vbase = reinterpret_cast<const char *>(staged);
-    const size_t view_bytes = view_texels * 4;
+    const char *vbase = reinterpret_cast<const char *>(J.staged);
+    const size_t view_bytes = J.view_texels * 4;
     const unsigned pitch4 = (unsigned)pitch_u * 4u;
-    const float fpitch4 = (float)pitch4;
-    const float fcol0 = (float)(pitch4 + 4u);
 
 #pragma unroll 1
-    for (int n = 0; n < n_views; ++n) {
-        const float4 ga = geom[3 * n + 0];
-        const float4 gb = geom[3 * n + 1];
-        const float4 gc = geom[3 * n + 2];
-        float vs = scale;
-        if (view_scale) vs *= view_scale[n];
+    for (int n = 0; n < J.n_views; ++n) {
+        const float4 ga = J.geom[3 * n + 0];
+        const float4 gb = J.geom[3 * n + 1];
+        const float4 gc = J.geom[3 * n + 2];
+        float vs = J.scale;
+        if (J.view_scale) vs *= J.view_scale[n];
 
-        // footprint of the tile: every lane evaluates the 8 corners with the
-        // same arithmetic as the voxels below, so the bounds are uniform
+        // footprint of the tile from its 4 x 2 corners, FP32 with a 0.05 px
+        // margin (the voxels below use FP64 coordinates); every lane computes
+        // the same bounds
         float umin = 3.0e38f, umax = -3.0e38f, vmin = 3.0e38f, vmax = -3.0e38f;
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
             for (int b = 0; b < 2; ++b) {
-                const float cden = fmaf(gc.x, ci[a], fmaf(gc.y, ck[b], gc.z));
-                const float cinv = __builtin_amdgcn_rcpf(cden);
-                const float cu = fmaf(ga.x, ci[a], fmaf(ga.y, ck[b], ga.z)) * cinv;
-                const float cvb = fmaf(gb.x, ci[a], fmaf(gb.y, ck[b], gb.z)) * cinv;
-                const float cdv = ga.w * cinv;
-                umin = fminf(umin, cu);
-                umax = fmaxf(umax, cu);
-#pragma unroll
-                for (int c = 0; c < 2; ++c) {
-                    const float cv = fmaf(cdv, cj[c], cvb);
-                    vmin = fminf(vmin, cv);
-                    vmax = fmaxf(vmax, cv);
-                }
+                float su, sva, svb;
+                corner_uv(ga, gb, gc, ci[a], ck[b], fj_a, fj_b, su, sva, svb);
+                umin = fminf(umin, su);
+                umax = fmaxf(umax, su);
+                vmin = fminf(vmin, fminf(sva, svb));
+                vmax = fmaxf(vmax, fmaxf(sva, svb));
             }
-        umin = __builtin_amdgcn_fmed3f(umin, -1.0f, fnu);
-        umax = __builtin_amdgcn_fmed3f(umax, -1.0f, fnu);
-        vmin = __builtin_amdgcn_fmed3f(vmin, -1.0f, fnv);
-        vmax = __builtin_amdgcn_fmed3f(vmax, -1.0f, fnv);
-        // detector texel ranges (inclusive) holding every tap, 0.01 px margin
-        const int u0 = __builtin_amdgcn_readfirstlane(max((int)floorf(umin - 0.01f), -1));
-        const int u1 = __builtin_amdgcn_readfirstlane(min((int)floorf(umax + 0.01f) + 1, nu + 1));
-        const int v0 = __builtin_amdgcn_readfirstlane(max((int)floorf(vmin - 0.01f), -1));
-        const int v1 = __builtin_amdgcn_readfirstlane(min((int)floorf(vmax + 0.01f) + 1, nv + 1));
+        umin = __builtin_amdgcn_fmed3f(umin, 0.0f, fnu + 1.0f);
+        umax = __builtin_amdgcn_fmed3f(umax, 0.0f, fnu + 1.0f);
+        vmin = __builtin_amdgcn_fmed3f(vmin, 0.0f, fnv + 1.0f);
+        vmax = __builtin_amdgcn_fmed3f(vmax, 0.0f, fnv + 1.0f);
+        const int u0 = __builtin_amdgcn_readfirstlane(max((int)floorf(umin - 0.05f), 0));
+        const int u1 = __builtin_amdgcn_readfirstlane(min((int)floorf(umax + 0.05f) + 1, nu + 2));
+        const int v0 = __builtin_amdgcn_readfirstlane(max((int)floorf(vmin - 0.05f), 0));
+        const int v1 = __builtin_amdgcn_readfirstlane(min((int)floorf(vmax + 0.05f) + 1, nv + 2));
         const int W = u1 - u0 + 1, H = v1 - v0 + 1;
 
-        const float den = fmaf(gc.x, fi, fmaf(gc.y, fk, gc.z));
-        const float inv = __builtin_amdgcn_rcpf(den);
-        float fu = fmaf(ga.x, fi, fmaf(ga.y, fk, ga.z)) * inv;
-        const float fvb = fmaf(gb.x, fi, fmaf(gb.y, fk, gb.z)) * inv;
-        const float dv = ga.w * inv;
-        const float w2 = inv * inv * vs;
-        fu = __builtin_amdgcn_fmed3f(fu, -1.0f, fnu);
-        const char *vptr = vbase + (size_t)n * view_bytes;
+        const ColumnView c = column_view64(J.geom64 + (size_t)kGeom64 * n, i, k, j0, vs);
+        const char *top = vbase + (size_t)n * view_bytes;
 
         if (W < kLdsCols && H < HMAX) {
-            // stage rows v0 .. v1 + 1 (one spare row, clamped into the view)
-            const unsigned *src = reinterpret_cast<const unsigned *>(vptr);
-            const int scol = min(u0 + 1 + lane, pitch_u - 1);
+            // stage staged rows v0 .. v1 + 1 (one spare row, clamped into the view)
+            const unsigned *src = reinterpret_cast<const unsigned *>(top);
+            const int scol = min(u0 + lane, pitch_u - 1);
             for (int r = wave; r <= H; r += 4) {
-                const int srow = min(v0 + 1 + r, pitch_v - 1);
+                const int srow = min(v0 + r, pitch_v - 1);
                 __builtin_amdgcn_global_load_lds(
                     (const void __attribute__((address_space(1))) *)(src + (size_t)srow * pitch_u + scol),
                     (void __attribute__((address_space(3))) *)(lds + r * kLdsPitch), 4, 0, 0);
@@ -351,57 +476,53 @@ __global__ __launch_bounds__(256) void backproject_lds_kernel(
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             __syncthreads();
 
-            const float ful = __builtin_amdgcn_fmed3f(fu - (float)u0, 0.0f, (float)(W - 1));
-            const float fu0 = floorf(ful);
-            const float wu = ful - fu0;
-            const unsigned colb = (unsigned)fu0 * 4u;
-            const float fvl = fmaf(dv, fj0, fvb) - (float)v0;
-            const float vhi = (float)(H - 1);
+            // rows clamped to the staged view and to the footprint
+            const Run R = make_run(c, (double)(W - 1), (double)u0, v0, min(nv + 1, v1), v0,
+                                   kLdsPitch * 4u, su_hi);
 #pragma unroll
             for (int t = 0; t < TY; ++t) {
-                const float fv = __builtin_amdgcn_fmed3f(fmaf(dv, (float)t, fvl), 0.0f, vhi);
-                const float fv0 = floorf(fv);
-                const float wv = fv - fv0;
-                const unsigned addr = __umul24((unsigned)fv0, kLdsPitch * 4u) + colb;
-                float2 top, bot;
-                lds_taps<DT>(lds, addr, top, bot);
-                const float ht = fmaf(wu, top.y - top.x, top.x);
-                const float hb = fmaf(wu, bot.y - bot.x, bot.x);
-                acc[t] = fmaf(w2, fmaf(wv, hb - ht, ht), acc[t]);
+                const float x = __builtin_amdgcn_fmed3f(fmaf(R.dv, (float)t, R.f), R.lo, R.hi);
+                const float wv = __builtin_amdgcn_fractf(x);
+                const unsigned addr = __umul24((unsigned)x, kLdsPitch * 4u) + R.base;
+                float2 a, b;
+                lds_taps<DT>(lds, addr, a, b);
+                const float left = fmaf(wv, b.x - a.x, a.x);
+                const float right = fmaf(wv, b.y - a.y, a.y);
+                acc[t] = fmaf(R.cR, right, fmaf(R.cL, left, acc[t]));
             }
             __syncthreads();
         } else {
-            const float fu0 = floorf(fu);
-            const float wu = fu - fu0;
-            const float fbase = fmaf(fu0, 4.0f, fcol0);
-            const float fv_j0 = fmaf(dv, fj0, fvb);
-#pragma unroll
-            for (int t = 0; t < TY; ++t) {
-                const float fv = __builtin_amdgcn_fmed3f(fmaf(dv, (float)t, fv_j0), -1.0f, fnv);
-                const float fv0 = floorf(fv);
-                const float wv = fv - fv0;
-                const unsigned off = (unsigned)fmaf(fv0, fpitch4, fbase);
-                const float2 top = Taps<DT>::load(vptr, off);
-                const float2 bot = Taps<DT>::load(vptr, off + pitch4);
-                const float ht = fmaf(wu, top.y - top.x, top.x);
-                const float hb = fmaf(wu, bot.y - bot.x, bot.x);
-                acc[t] = fmaf(w2, fmaf(wv, hb - ht, ht), acc[t]);
-            }
+            const Run R = make_run(c, su_hi, 0.0, 0, nv + 1, 0, pitch4, su_hi);
+            gather_run<DT, TY>(acc, R, top, top + pitch4, pitch4);
         }
     }
+    store_run<TY>(acc, J.vol, i, j0, k, J.X, J.Y, J.Z, J.accumulate, J.clip);
+}
 
-    if (i >= X || k >= Z) return;
-#pragma unroll
-    for (int t = 0; t < TY; ++t) {
-        const int j = j0 + t;
-        if (j < Y) {
-            const size_t idx = ((size_t)k * Y + j) * X + i;
-            float v = acc[t];
-            if (accumulate) v += vol[idx];
-            if (clip) v = fmaxf(v, 0.0f);
-            vol[idx] = v;
-        }
-    }
+template <int DT, int TY, int HMAX>
+__global__ __launch_bounds__(256) void backproject_lds_kernel(BpJob J)
+{
+    __shared__ unsigned lds[HMAX * kLdsPitch];
+    backproject_lds_tile<DT, TY, HMAX>(J, blockIdx.x, blockIdx.y, blockIdx.z, lds);
+}
+
+// Packed launch: the flat block index selects the job (the last job whose
+// block_begin it reaches) and the tile inside it.
+template <int DT, int TY, int HMAX>
+__global__ __launch_bounds__(256) void backproject_lds_packed_kernel(BpBatch B)
+{
+    __shared__ unsigned lds[HMAX * kLdsPitch];
+    const int b = blockIdx.x;
+    int j = 0;
+#pragma unroll 1
+    for (int q = 1; q < B.count; ++q)
+        if (b >= B.job[q].block_begin) j = q;
+    const BpJob &J = B.job[j];
+    const int local = b - J.block_begin;
+    const int bx = local % J.gx;
+    const int by = (local / J.gx) % J.gy;
+    const int bz = local / (J.gx * J.gy);
+    backproject_lds_tile<DT, TY, HMAX>(J, bx, by, bz, lds);
 }
 
 // Kernel choice: the LDS-staged kernel for volumes at least 32 voxels along
@@ -416,28 +537,51 @@ static int variant_override()
     return 0;
 }
 
+constexpr int kTY = 32, kHMAX = 112;  // LDS kernel: y-run and staged rows
+
+static BpJob make_job(const void *staged, const StagedLayout &L, int nu, int nv, GeomPtrs geom,
+                      const float *view_scale, int n_views, float *vol, int X, int Y, int Z,
+                      float scale, int accumulate, int clip)
+{
+    BpJob J;
+    J.staged = staged;
+    J.geom = geom.f32;
+    J.geom64 = geom.f64;
+    J.view_scale = view_scale;
+    J.vol = vol;
+    J.view_texels = L.view;
+    J.pitch_u = L.pitch_u;
+    J.pitch_v = L.pitch_v;
+    J.nu = nu;
+    J.nv = nv;
+    J.n_views = n_views;
+    J.X = X;
+    J.Y = Y;
+    J.Z = Z;
+    J.scale = scale;
+    J.accumulate = accumulate;
+    J.clip = clip;
+    J.gx = (X + 15) / 16;
+    J.gy = (Z + 15) / 16;
+    J.block_begin = 0;
+    return J;
+}
+
 template <int DT>
-static int launch_bp(hipStream_t s, int vy, const void *staged, const StagedLayout &L, int nu,
-                     int nv, const float4 *geom, const float *view_scale, int n_views,
-                     float *vol, int X, int Y, int Z, float scale, int accumulate, int clip)
+static int launch_bp(hipStream_t s, const BpJob &J)
 {
     const int ov = variant_override();
-    if ((ov == 2 || (ov == 0 && Y >= 32))) {
-        constexpr int TY = 32, HMAX = 112;
-        dim3 grid((X + 15) / 16, (Z + 15) / 16, (Y + TY - 1) / TY);
-        backproject_lds_kernel<DT, TY, HMAX><<<grid, 256, 0, s>>>(
-            staged, L.pitch_u, L.pitch_v, L.view, nu, nv, geom, view_scale, n_views, vol, X, Y,
-            Z, scale, accumulate, clip);
+    if (ov == 2 || (ov == 0 && J.Y >= 32)) {
+        dim3 grid(J.gx, J.gy, (J.Y + kTY - 1) / kTY);
+        backproject_lds_kernel<DT, kTY, kHMAX><<<grid, 256, 0, s>>>(J);
         CTR_CHECK_LAUNCH();
         return CTR_OK;
     }
-    dim3 block(256);
+    const int vy = J.Y >= 16 ? 16 : J.Y >= 8 ? 8 : 1;
 #define CTR_BP_CASE(V)                                                                   \
     case V: {                                                                            \
-        dim3 grid((X + 15) / 16, (Z + 15) / 16, (Y + V - 1) / V);                        \
-        backproject_kernel<DT, V><<<grid, block, 0, s>>>(                                \
-            staged, L.pitch_u, L.view, (float)nu, (float)nv, geom, view_scale, n_views,  \
-            vol, X, Y, Z, scale, accumulate, clip);                                      \
+        dim3 grid(J.gx, J.gy, (J.Y + V - 1) / V);                                        \
+        backproject_kernel<DT, V><<<grid, 256, 0, s>>>(J);                               \
         break;                                                                           \
     }
     switch (vy) {
@@ -482,23 +626,87 @@ int ctr_backproject(void *stream, ctr_workspace *ws, int geometry, const void *s
     ctr_workspace *w = nullptr;
     if ((rc = workspace_resolve(ws, &w))) return rc;
     hipStream_t s = as_stream(stream);
-    const float4 *geom = nullptr;
-    if ((rc = build_geometry(s, w, geometry, n_views, source_distance, detector_distance,
-                             detector_width, volume_width, rotation_count, rotation_offset, X,
-                             Y, Z, nu, nv, &geom)))
+    void *p = nullptr;
+    if ((rc = workspace_get(w, SLOT_GEOMETRY, kGeomBytesPerView * (size_t)n_views, &p)))
         return rc;
-    int vy = Y >= 16 ? 16 : (Y >= 8 ? 8 : 1);
+    GeomPtrs g;
+    if ((rc = build_geometry_at(s, p, geometry, n_views, source_distance, detector_distance,
+                                detector_width, volume_width, rotation_count, rotation_offset, X,
+                                Y, Z, nu, nv, &g)))
+        return rc;
+    const BpJob J = make_job(staged, L, nu, nv, g, view_scale, n_views, volume, X, Y, Z, scale,
+                             accumulate, clip_negative);
+    switch (staged_dtype) {
+    case CTR_F32: return launch_bp<CTR_F32>(s, J);
+    case CTR_F16: return launch_bp<CTR_F16>(s, J);
+    default: return launch_bp<CTR_BF16>(s, J);
+    }
+}
+
+int ctr_backproject_packed(void *stream, ctr_workspace *ws, int staged_dtype,
+                           const ctr_backproject_job *jobs, int count)
+{
+    clear_error();
+    CTR_REQUIRE(jobs, CTR_ERR_NULL_POINTER, "ctr_backproject_packed: jobs is NULL");
+    CTR_REQUIRE(count >= 1 && count <= kMaxPacked, CTR_ERR_INVALID_ARGUMENT,
+                "ctr_backproject_packed: count %d outside [1, %d]", count, kMaxPacked);
+    CTR_REQUIRE(staged_dtype == CTR_F32 || staged_dtype == CTR_F16 || staged_dtype == CTR_BF16,
+                CTR_ERR_INVALID_ARGUMENT, "unknown staged dtype %d", staged_dtype);
+    size_t total_views = 0;
+    for (int q = 0; q < count; q++) {
+        const ctr_backproject_job &j = jobs[q];
+        CTR_REQUIRE(j.staged && j.volume, CTR_ERR_NULL_POINTER,
+                    "ctr_backproject_packed: job %d has a NULL buffer", q);
+        CTR_REQUIRE(staged_dtype == CTR_F32 || j.view_scale, CTR_ERR_NULL_POINTER,
+                    "ctr_backproject_packed: job %d needs view_scale", q);
+        int rc = check_geometry(j.geometry, j.source_distance, j.detector_distance,
+                                j.detector_width, j.volume_width, j.X, j.Y, j.Z, j.nu, j.nv,
+                                j.n_views);
+        if (rc) return rc;
+        StagedLayout L = staged_layout(j.nu, j.nv, j.n_views, staged_dtype);
+        CTR_REQUIRE(L.view * 4 < ((size_t)1 << 24), CTR_ERR_INVALID_ARGUMENT,
+                    "job %d: staged view too large", q);
+        total_views += (size_t)j.n_views;
+    }
+    ctr_workspace *w = nullptr;
+    int rc = workspace_resolve(ws, &w);
+    if (rc) return rc;
+    void *p = nullptr;
+    if ((rc = workspace_get(w, SLOT_GEOMETRY, kGeomBytesPerView * total_views, &p))) return rc;
+    hipStream_t s = as_stream(stream);
+    BpBatch B;
+    B.count = count;
+    char *g = (char *)p;
+    long long blocks = 0;
+    for (int q = 0; q < count; q++) {
+        const ctr_backproject_job &j = jobs[q];
+        GeomPtrs gp;
+        if ((rc = build_geometry_at(s, g, j.geometry, j.n_views, j.source_distance,
+                                    j.detector_distance, j.detector_width, j.volume_width,
+                                    j.rotation_count, j.rotation_offset, j.X, j.Y, j.Z, j.nu,
+                                    j.nv, &gp)))
+            return rc;
+        StagedLayout L = staged_layout(j.nu, j.nv, j.n_views, staged_dtype);
+        B.job[q] = make_job(j.staged, L, j.nu, j.nv, gp, j.view_scale, j.n_views, j.volume, j.X,
+                            j.Y, j.Z, j.scale, j.accumulate, j.clip_negative);
+        B.job[q].block_begin = (int)blocks;
+        blocks += (long long)B.job[q].gx * B.job[q].gy * ((j.Y + kTY - 1) / kTY);
+        g += kGeomBytesPerView * (size_t)j.n_views;
+    }
+    CTR_REQUIRE(blocks < (1LL << 31), CTR_ERR_INVALID_ARGUMENT, "packed launch too large");
     switch (staged_dtype) {
     case CTR_F32:
-        return launch_bp<CTR_F32>(s, vy, staged, L, nu, nv, geom, view_scale, n_views, volume,
-                                  X, Y, Z, scale, accumulate, clip_negative);
+        backproject_lds_packed_kernel<CTR_F32, kTY, kHMAX><<<(unsigned)blocks, 256, 0, s>>>(B);
+        break;
     case CTR_F16:
-        return launch_bp<CTR_F16>(s, vy, staged, L, nu, nv, geom, view_scale, n_views, volume,
-                                  X, Y, Z, scale, accumulate, clip_negative);
+        backproject_lds_packed_kernel<CTR_F16, kTY, kHMAX><<<(unsigned)blocks, 256, 0, s>>>(B);
+        break;
     default:
-        return launch_bp<CTR_BF16>(s, vy, staged, L, nu, nv, geom, view_scale, n_views, volume,
-                                   X, Y, Z, scale, accumulate, clip_negative);
+        backproject_lds_packed_kernel<CTR_BF16, kTY, kHMAX><<<(unsigned)blocks, 256, 0, s>>>(B);
+        break;
     }
+    CTR_CHECK_LAUNCH();
+    return CTR_OK;
 }
 
 }  // extern "C"

@@ -68,6 +68,10 @@ struct StagedLayout {
 };
 StagedLayout staged_layout(int nu, int nv, int n_views, int dtype);
 
+// Workspace bytes of per-view geometry: 3 float4 (FP32 rows for tile
+// footprints) + 12 doubles (FP64 rows for voxel coordinates).
+constexpr size_t kGeometryBytesPerView = 3 * 16 + 12 * 8;
+
 // Per-view projective rows in voxel-index space (built in FP64 on device).
 // rec[3n+0] = (U_i, U_k, U_0, V_j), rec[3n+1] = (V_i, V_k, V_0, 0),
 // rec[3n+2] = (D_i, D_k, D_0, 0):

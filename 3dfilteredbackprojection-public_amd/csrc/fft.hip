@@ -285,6 +285,243 @@ __global__ __launch_bounds__(kFftThreads) void filter_pairs_kernel(FilterJob J)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Stockham radix-8 row filter (n >= 512)
+// ---------------------------------------------------------------------------
+// n/8 threads per row pair.  Every thread keeps 8 complex values in
+// registers through a pass and exchanges them through LDS between passes
+// (read, barrier, radix-R butterflies, write, barrier): radix-8 passes plus
+// a final radix-2 or radix-4 pass when log2 n is not a multiple of 3, so a
+// 2048-point transform takes 4 LDS round trips instead of 11.  The first
+// forward pass reads the detector rows straight from memory, the gain is
+// applied as the inverse transform reads the spectrum, and the last inverse
+// pass writes the cropped rows straight to memory.  LDS indices carry one
+// float2 of padding per 32 against bank conflicts of the stride-8 writes.
+
+__device__ inline int lpad(int i) { return i + (i >> 5); }
+
+template <bool INV>
+__device__ inline float2 times_minus_i(float2 a)  // a * (-i) forward, a * (+i) inverse
+{
+    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+__device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+template <int R, bool INV>
+struct SmallFFT;
+
+template <bool INV>
+struct SmallFFT<2, INV> {
+    __device__ static inline void run(float2 *x)
+    {
+        float2 a = x[0], b = x[1];
+        x[0] = cadd(a, b);
+        x[1] = csub(a, b);
+    }
+};
+
+template <bool INV>
+struct SmallFFT<4, INV> {
+    __device__ static inline void run(float2 *x)
+    {
+        float2 y0 = cadd(x[0], x[2]), y1 = csub(x[0], x[2]);
+        float2 y2 = cadd(x[1], x[3]), y3 = times_minus_i<INV>(csub(x[1], x[3]));
+        x[0] = cadd(y0, y2);
+        x[2] = csub(y0, y2);
+        x[1] = cadd(y1, y3);
+        x[3] = csub(y1, y3);
+    }
+};
+
+template <bool INV>
+struct SmallFFT<8, INV> {
+    __device__ static inline void run(float2 *x)
+    {
+        const float h = 0.70710678118654752440f;  // 1/sqrt(2)
+        float2 e[4] = {x[0], x[2], x[4], x[6]};
+        float2 o[4] = {x[1], x[3], x[5], x[7]};
+        SmallFFT<4, INV>::run(e);
+        SmallFFT<4, INV>::run(o);
+        // o_k *= exp(-+2 pi i k / 8)
+        float2 a = o[1];
+        o[1] = INV ? make_float2((a.x - a.y) * h, (a.x + a.y) * h)
+                   : make_float2((a.x + a.y) * h, (a.y - a.x) * h);
+        o[2] = times_minus_i<INV>(o[2]);
+        a = o[3];
+        o[3] = INV ? make_float2(-(a.x + a.y) * h, (a.x - a.y) * h)
+                   : make_float2((a.y - a.x) * h, -(a.x + a.y) * h);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            x[k] = cadd(e[k], o[k]);
+            x[k + 4] = csub(e[k], o[k]);
+        }
+    }
+};
+
+// Radix-R butterflies of one Stockham pass with span Ns on the 8 / R
+// butterflies this thread owns (v[b * R + r]).
+template <int N, int R, bool INV>
+__device__ inline void stockham_butterflies(float2 (&v)[8], int Ns, const float2 *Tw, int tid)
+{
+    constexpr int T = N / 8, B = 8 / R;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int j = tid + b * T;
+        const int jm = j & (Ns - 1);
+        if (Ns > 1) {
+            const int step = (N / R) / Ns;
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                float2 w = Tw[jm * r * step];
+                if (INV) w.y = -w.y;
+                v[b * R + r] = cmul(v[b * R + r], w);
+            }
+        }
+        SmallFFT<R, INV>::run(&v[b * R]);
+    }
+}
+
+template <int N, int R>
+__device__ inline void stockham_read(float2 (&v)[8], const float2 *L, int tid)
+{
+    constexpr int T = N / 8, B = 8 / R;
+#pragma unroll
+    for (int b = 0; b < B; ++b)
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[b * R + r] = L[lpad(tid + b * T + r * (N / R))];
+}
+
+template <int N, int R>
+__device__ inline void stockham_write(const float2 (&v)[8], float2 *L, int Ns, int tid)
+{
+    constexpr int T = N / 8, B = 8 / R;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int j = tid + b * T;
+        const int jm = j & (Ns - 1);
+        const int base = (j - jm) * R + jm;
+#pragma unroll
+        for (int r = 0; r < R; ++r) L[lpad(base + r * Ns)] = v[b * R + r];
+    }
+}
+
+// One full in-LDS pass (read, barrier, butterflies, write, barrier).
+template <int N, int R, bool INV>
+__device__ inline void stockham_pass(float2 *L, const float2 *Tw, int Ns, int tid)
+{
+    float2 v[8];
+    stockham_read<N, R>(v, L, tid);
+    __syncthreads();
+    stockham_butterflies<N, R, INV>(v, Ns, Tw, tid);
+    stockham_write<N, R>(v, L, Ns, tid);
+    __syncthreads();
+}
+
+template <int LOG2N>
+__global__ __launch_bounds__((1 << LOG2N) / 8) void filter_stockham_kernel(FilterJob J)
+{
+    constexpr int N = 1 << LOG2N, T = N / 8;
+    constexpr int P8 = LOG2N / 3;             // radix-8 passes
+    constexpr int RL = 1 << (LOG2N % 3);      // final radix (1: none)
+    static_assert(P8 >= 2, "the Stockham filter needs n >= 64");
+    __shared__ float2 L[N + N / 32];
+    __shared__ float2 Tw[N];
+    const int tid = threadIdx.x;
+
+    // full-circle twiddle table exp(-2 pi i k / N)
+    for (int k = tid; k < N / 2; k += T) {
+        float2 w = g_twiddle[k * (kMaxN / N)];
+        Tw[k] = w;
+        Tw[k + N / 2] = make_float2(-w.x, -w.y);
+    }
+
+    const int r0 = 2 * blockIdx.x, r1 = r0 + 1;
+    const bool has1 = r1 < J.rows;
+    int view0 = 0, iv0 = 0, view1 = 0, iv1 = 0;
+    row_coords(J, r0, view0, iv0);
+    if (has1) row_coords(J, r1, view1, iv1);
+    double vv0 = 0.0, vv1 = 0.0, sdd = 0.0;
+    const double cu = 0.5 * J.width - 0.5;
+    if (J.fdk) {
+        double v0 = (iv0 + 0.5 - 0.5 * J.nv) * J.delta;
+        double v1 = (iv1 + 0.5 - 0.5 * J.nv) * J.delta;
+        vv0 = J.sdd2 + v0 * v0;
+        vv1 = J.sdd2 + v1 * v1;
+        sdd = sqrt(J.sdd2);
+    }
+    const float *a_row = J.in + (size_t)r0 * J.width;
+    const float *b_row = J.in + (size_t)r1 * J.width;
+
+    // forward pass 0 (radix 8, span 1) straight from the detector rows
+    float2 v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int e = tid + r * T;
+        float a = 0.f, b = 0.f;
+        if (e < J.width) {
+            a = a_row[e];
+            if (has1) b = b_row[e];
+            if (J.fdk) {
+                double u = (e - cu) * J.delta;
+                a = (float)((double)a * (sdd / sqrt(vv0 + u * u)));
+                b = (float)((double)b * (sdd / sqrt(vv1 + u * u)));
+            }
+        }
+        v[r] = make_float2(a, b);
+    }
+    SmallFFT<8, false>::run(v);
+    stockham_write<N, 8>(v, L, 1, tid);
+    __syncthreads();
+    int Ns = 8;
+#pragma unroll 1
+    for (int p = 1; p < P8; ++p, Ns *= 8) stockham_pass<N, 8, false>(L, Tw, Ns, tid);
+    if (RL > 1) stockham_pass<N, (RL > 1 ? RL : 2), false>(L, Tw, Ns, tid);
+
+    // inverse pass 0 (radix 8, span 1) with the gain applied on read
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int e = tid + r * T;
+        const float g = unshifted_gain(J.filter, e, N);
+        const float2 x = L[lpad(e)];
+        v[r] = make_float2(x.x * g, x.y * g);
+    }
+    __syncthreads();
+    SmallFFT<8, true>::run(v);
+    stockham_write<N, 8>(v, L, 1, tid);
+    __syncthreads();
+    Ns = 8;
+    constexpr int mid_end = RL > 1 ? P8 : P8 - 1;  // in-LDS radix-8 passes
+#pragma unroll 1
+    for (int p = 1; p < mid_end; ++p, Ns *= 8) stockham_pass<N, 8, true>(L, Tw, Ns, tid);
+
+    // last inverse pass: span N/R, so element index = j + r * N/R
+    constexpr int RF = RL > 1 ? RL : 8;
+    stockham_read<N, RF>(v, L, tid);
+    stockham_butterflies<N, RF, true>(v, Ns, Tw, tid);
+    const float inv_n = 1.0f / (float)N;
+    float *oa, *ob;
+    if (J.staged == nullptr) {
+        oa = J.out + (size_t)r0 * J.width;
+        ob = J.out + (size_t)r1 * J.width;
+    } else {
+        size_t vstride = (size_t)J.pitch_u * J.pitch_v;
+        oa = J.staged + (size_t)view0 * vstride + (size_t)(iv0 + 1) * J.pitch_u + 1;
+        ob = J.staged + (size_t)view1 * vstride + (size_t)(iv1 + 1) * J.pitch_u + 1;
+    }
+#pragma unroll
+    for (int b = 0; b < 8 / RF; ++b)
+#pragma unroll
+        for (int r = 0; r < RF; ++r) {
+            const int e = tid + b * T + r * (N / RF);
+            if (e < J.width) {
+                oa[e] = v[b * RF + r].x * inv_n;
+                if (has1) ob[e] = v[b * RF + r].y * inv_n;
+            }
+        }
+}
+
 static int launch_filter_pairs(hipStream_t s, const FilterJob &J, int log2n)
 {
     int blocks = (J.rows + 1) / 2;
@@ -293,15 +530,21 @@ static int launch_filter_pairs(hipStream_t s, const FilterJob &J, int log2n)
     case L:                                                                       \
         filter_pairs_kernel<L><<<blocks, kFftThreads, 0, s>>>(J);                 \
         break;
+#define CTR_STOCK_CASE(L)                                                         \
+    case L:                                                                       \
+        filter_stockham_kernel<L><<<blocks, (1 << L) / 8, 0, s>>>(J);             \
+        break;
     switch (log2n) {
         CTR_FILT_CASE(0) CTR_FILT_CASE(1) CTR_FILT_CASE(2) CTR_FILT_CASE(3)
         CTR_FILT_CASE(4) CTR_FILT_CASE(5) CTR_FILT_CASE(6) CTR_FILT_CASE(7)
-        CTR_FILT_CASE(8) CTR_FILT_CASE(9) CTR_FILT_CASE(10) CTR_FILT_CASE(11)
-        CTR_FILT_CASE(12) CTR_FILT_CASE(13)
+        CTR_FILT_CASE(8)
+        CTR_STOCK_CASE(9) CTR_STOCK_CASE(10) CTR_STOCK_CASE(11) CTR_STOCK_CASE(12)
+        CTR_STOCK_CASE(13)
     default:
         return fail(CTR_ERR_NOT_POWER_OF_TWO, "FFT length 2^%d unsupported", log2n);
     }
 #undef CTR_FILT_CASE
+#undef CTR_STOCK_CASE
     CTR_CHECK_LAUNCH();
     return CTR_OK;
 }

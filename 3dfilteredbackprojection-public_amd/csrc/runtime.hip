@@ -138,7 +138,7 @@ int ctr_workspace_reserve(ctr_workspace *ws, int n_views, int max_fft_length)
     if (rc) return rc;
     void *p = nullptr;
     if ((rc = ctr::workspace_get(w, ctr::SLOT_GEOMETRY,
-                                 sizeof(float4) * 3 * (size_t)(n_views + 1), &p)))
+                                 ctr::kGeometryBytesPerView * (size_t)(n_views + 1), &p)))
         return rc;
     if ((rc = ctr::workspace_get(w, ctr::SLOT_VIEW_MAX,
                                  sizeof(unsigned) * (size_t)(n_views + 1), &p)))

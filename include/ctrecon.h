@@ -206,6 +206,27 @@ int ctr_backproject(void *stream, ctr_workspace *ws, int geometry,
                     float *volume, int X, int Y, int Z, float scale,
                     int accumulate, int clip_negative);
 
+/* One reconstruction of a packed backprojection (fields as in
+ * ctr_backproject). */
+typedef struct ctr_backproject_job {
+    const void *staged;
+    const float *view_scale;
+    float *volume;
+    int geometry, nu, nv, n_views, X, Y, Z;
+    double source_distance, detector_distance, detector_width, volume_width;
+    double rotation_count, rotation_offset;
+    float scale;
+    int accumulate, clip_negative;
+} ctr_backproject_job;
+
+/* Up to 16 independent backprojections (any sizes and geometries, one
+ * staged dtype) in ONE persistent launch: the serving loop packs small
+ * reconstructions this way so that they fill the GPU together.  Each job's
+ * result equals ctr_backproject on that job alone.  No reference
+ * counterpart (the reference reconstructs one volume at a time). */
+int ctr_backproject_packed(void *stream, ctr_workspace *ws, int staged_dtype,
+                           const ctr_backproject_job *jobs, int count);
+
 /* ---- forward projection ------------------------------------------------------
  * Ray-driven projector, one ray per detector pixel centre, replacing
  *   FBP3D::project_forward_cone_to_projections   (FBP3D.cpp:269-299)

@@ -113,17 +113,20 @@ def test_full_size_filter_rows_sample(ctr, orc, dev):
 
 
 def test_backproject_kernel_variants_agree(ctr, dev, monkeypatch):
-    """The LDS-staged kernel and the direct-tap kernel give the same volume."""
+    """The LDS-staged kernel and the direct-tap kernel give bit-identical
+    volumes: both take their coordinates from the same FP64 column set-up and
+    the LDS footprint holds exactly the taps the direct kernel reads."""
     import torch
     g = ctr.config_geometry("fdk_256_s")
     q = _random_views(ctr, g, dev, 5)
-    staged, _ = ctr.stage_projections(q, g, ctr.F32)
-    monkeypatch.setenv("CTR_BACKPROJECT_KERNEL", "lds")
-    a = ctr.backproject(staged, g, ctr.F32, scale=0.01)
-    monkeypatch.setenv("CTR_BACKPROJECT_KERNEL", "gather")
-    b = ctr.backproject(staged, g, ctr.F32, scale=0.01)
-    torch.cuda.synchronize()
-    assert rel_max(a.cpu().numpy(), b.cpu().numpy()) < 2e-6
+    for dt in (ctr.F32, ctr.F16):
+        staged, vs = ctr.stage_projections(q, g, dt)
+        monkeypatch.setenv("CTR_BACKPROJECT_KERNEL", "lds")
+        a = ctr.backproject(staged, g, dt, view_scale=vs, scale=0.01)
+        monkeypatch.setenv("CTR_BACKPROJECT_KERNEL", "gather")
+        b = ctr.backproject(staged, g, dt, view_scale=vs, scale=0.01)
+        torch.cuda.synchronize()
+        assert torch.equal(a, b)
 
 
 def test_lds_kernel_falls_back_on_wide_footprints(ctr, orc, dev, monkeypatch):
