@@ -110,6 +110,34 @@ int check_geometry(int geometry, double sod, double odd, double detector_width,
     return CTR_OK;
 }
 
+// Detector footprint of every 16 x 16 column tile of the LDS kernel, per
+// view, from the tile's 4 corner columns in FP64: (min su, max su, min 1/den,
+// max 1/den) in staged coordinates.  For a run of rows j the staged v range
+// follows as cv + 1 + Vj (j + oy) [min, max] 1/den, because
+// fv = cv + Vj (j + oy) / den for this parameterisation (cone and parallel).
+__global__ void footprint_kernel(const double *g64, int n_views, int gx, int gy, float4 *out)
+{
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long total = (long long)n_views * gx * gy;
+    if (idx >= total) return;
+    const int bx = (int)(idx % gx);
+    const int by = (int)((idx / gx) % gy);
+    const int n = (int)(idx / ((long long)gx * gy));
+    const double *g = g64 + (size_t)kGeom64 * n;
+    double umin = 1e300, umax = -1e300, imin = 1e300, imax = -1e300;
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+            const double ci = bx * 16 + 15 * a, ck = by * 16 + 15 * b;
+            const double inv = 1.0 / fma(g[7], ci, fma(g[8], ck, g[9]));
+            const double su = fma(g[0], ci, fma(g[1], ck, g[2])) * inv + 1.0;
+            umin = fmin(umin, su);
+            umax = fmax(umax, su);
+            imin = fmin(imin, inv);
+            imax = fmax(imax, inv);
+        }
+    out[idx] = make_float4((float)umin, (float)umax, (float)imin, (float)imax);
+}
+
 // Bytes of geometry records per view (FP32 footprint rows + FP64 rows).
 constexpr size_t kGeomBytesPerView = kGeometryBytesPerView;
 static_assert(kGeometryBytesPerView == sizeof(float4) * 3 + sizeof(double) * kGeom64, "layout");
@@ -294,6 +322,7 @@ struct BpJob {
     const void *staged;
     const float4 *geom;
     const double *geom64;
+    const float4 *foot;  // footprint table (LDS kernel), gx * gy per view
     const float *view_scale;
     float *vol;
     size_t view_texels;
@@ -303,6 +332,9 @@ struct BpJob {
     int accumulate, clip;
     int gx, gy;       // tiles along x and z
     int block_begin;  // first flat block of this job in a packed launch
+    // LDS row pitch in bytes, passed at run time: with a compile-time
+    // constant the compiler widens the 24-bit row mad into a 64-bit one
+    unsigned lds_pitch4;
 };
 
 constexpr int kMaxPacked = 16;
@@ -388,20 +420,6 @@ __device__ inline void lds_taps<CTR_BF16>(const unsigned *L, unsigned addr, floa
     bot = bf16_pair(P[kLdsPitch]);
 }
 
-// FP32 staged u and v of a voxel from the FP32 rows (footprints only).
-__device__ inline void corner_uv(const float4 &ga, const float4 &gb, const float4 &gc, float fi,
-                                 float fk, float fj_a, float fj_b, float &su, float &sva,
-                                 float &svb)
-{
-    const float den = fmaf(gc.x, fi, fmaf(gc.y, fk, gc.z));
-    const float inv = __builtin_amdgcn_rcpf(den);
-    su = fmaf(ga.x, fi, fmaf(ga.y, fk, ga.z)) * inv + 1.0f;
-    const float fvb = fmaf(gb.x, fi, fmaf(gb.y, fk, gb.z)) * inv + 1.0f;
-    const float dv = ga.w * inv;
-    sva = fmaf(dv, fj_a, fvb);
-    svb = fmaf(dv, fj_b, fvb);
-}
-
 template <int DT, int TY, int HMAX>
 __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int bz,
                                             unsigned *lds)
@@ -414,10 +432,12 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
     const int j0 = bz * TY;
     const float fnu = (float)nu, fnv = (float)nv;
     const double su_hi = (double)nu + 1.0;
-    // tile corners (voxel-centre indices) for the footprint
-    const float ci[2] = {(float)ti0, (float)(ti0 + 15)};
-    const float ck[2] = {(float)tk0, (float)(tk0 + 15)};
-    const float fj_a = (float)j0, fj_b = (float)(j0 + TY - 1);
+    // rows of the run relative to the volume centre, and the staged v centre
+    const float oy = 0.5f - 0.5f * (float)J.Y;
+    const float ja = (float)j0 + oy, jb = (float)(j0 + TY - 1) + oy;
+    const float cv1 = 0.5f * fnv + 0.5f;
+    const float4 *foot = J.foot + (size_t)by * J.gx + bx;
+    const size_t foot_stride = (size_t)J.gx * J.gy;
 
     float acc[TY];
 #pragma unroll
@@ -429,27 +449,18 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
 
 #pragma unroll 1
     for (int n = 0; n < J.n_views; ++n) {
-        const float4 ga = J.geom[3 * n + 0];
-        const float4 gb = J.geom[3 * n + 1];
-        const float4 gc = J.geom[3 * n + 2];
         float vs = J.scale;
         if (J.view_scale) vs *= J.view_scale[n];
 
-        // footprint of the tile from its 4 x 2 corners, FP32 with a 0.05 px
-        // margin (the voxels below use FP64 coordinates); every lane computes
-        // the same bounds
-        float umin = 3.0e38f, umax = -3.0e38f, vmin = 3.0e38f, vmax = -3.0e38f;
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                float su, sva, svb;
-                corner_uv(ga, gb, gc, ci[a], ck[b], fj_a, fj_b, su, sva, svb);
-                umin = fminf(umin, su);
-                umax = fmaxf(umax, su);
-                vmin = fminf(vmin, fminf(sva, svb));
-                vmax = fmaxf(vmax, fmaxf(sva, svb));
-            }
+        // footprint of the tile: u range and 1/den range from the table, the
+        // v range of this run of rows from them (0.05 px margin below; the
+        // voxels use FP64 coordinates)
+        const float4 fp = foot[(size_t)n * foot_stride];
+        const float vj = J.geom[3 * n].w;
+        const float ya = ja * vj, yb = jb * vj;
+        float umin = fp.x, umax = fp.y;
+        float vmin = cv1 + fminf(fminf(ya * fp.z, ya * fp.w), fminf(yb * fp.z, yb * fp.w));
+        float vmax = cv1 + fmaxf(fmaxf(ya * fp.z, ya * fp.w), fmaxf(yb * fp.z, yb * fp.w));
         umin = __builtin_amdgcn_fmed3f(umin, 0.0f, fnu + 1.0f);
         umax = __builtin_amdgcn_fmed3f(umax, 0.0f, fnu + 1.0f);
         vmin = __builtin_amdgcn_fmed3f(vmin, 0.0f, fnv + 1.0f);
@@ -477,13 +488,14 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
             __syncthreads();
 
             // rows clamped to the staged view and to the footprint
-            const Run R = make_run(c, (double)(W - 1), (double)u0, v0, min(nv + 1, v1), v0,
-                                   kLdsPitch * 4u, su_hi);
+            const unsigned lp4 = J.lds_pitch4;
+            const Run R = make_run(c, (double)(W - 1), (double)u0, v0, min(nv + 1, v1), v0, lp4,
+                                   su_hi);
 #pragma unroll
             for (int t = 0; t < TY; ++t) {
                 const float x = __builtin_amdgcn_fmed3f(fmaf(R.dv, (float)t, R.f), R.lo, R.hi);
                 const float wv = __builtin_amdgcn_fractf(x);
-                const unsigned addr = __umul24((unsigned)x, kLdsPitch * 4u) + R.base;
+                const unsigned addr = __umul24((unsigned)x, lp4) + R.base;
                 float2 a, b;
                 lds_taps<DT>(lds, addr, a, b);
                 const float left = fmaf(wv, b.x - a.x, a.x);
@@ -563,15 +575,38 @@ static BpJob make_job(const void *staged, const StagedLayout &L, int nu, int nv,
     J.clip = clip;
     J.gx = (X + 15) / 16;
     J.gy = (Z + 15) / 16;
+    J.foot = nullptr;
     J.block_begin = 0;
+    J.lds_pitch4 = kLdsPitch * 4u;
     return J;
+}
+
+static bool use_lds_kernel(int Y)
+{
+    const int ov = variant_override();
+    return ov == 2 || (ov == 0 && Y >= 32);
+}
+
+// Fills the footprint table of job J at `dst` (gx * gy * n_views float4).
+static int build_footprints(hipStream_t s, BpJob &J, float4 *dst)
+{
+    const long long total = (long long)J.gx * J.gy * J.n_views;
+    footprint_kernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(J.geom64, J.n_views, J.gx,
+                                                                      J.gy, dst);
+    CTR_CHECK_LAUNCH();
+    J.foot = dst;
+    return CTR_OK;
+}
+
+static size_t footprint_bytes(int X, int Z, int n_views)
+{
+    return sizeof(float4) * (size_t)((X + 15) / 16) * ((Z + 15) / 16) * n_views;
 }
 
 template <int DT>
 static int launch_bp(hipStream_t s, const BpJob &J)
 {
-    const int ov = variant_override();
-    if (ov == 2 || (ov == 0 && J.Y >= 32)) {
+    if (use_lds_kernel(J.Y)) {
         dim3 grid(J.gx, J.gy, (J.Y + kTY - 1) / kTY);
         backproject_lds_kernel<DT, kTY, kHMAX><<<grid, 256, 0, s>>>(J);
         CTR_CHECK_LAUNCH();
@@ -634,8 +669,13 @@ int ctr_backproject(void *stream, ctr_workspace *ws, int geometry, const void *s
                                 detector_width, volume_width, rotation_count, rotation_offset, X,
                                 Y, Z, nu, nv, &g)))
         return rc;
-    const BpJob J = make_job(staged, L, nu, nv, g, view_scale, n_views, volume, X, Y, Z, scale,
-                             accumulate, clip_negative);
+    BpJob J = make_job(staged, L, nu, nv, g, view_scale, n_views, volume, X, Y, Z, scale,
+                       accumulate, clip_negative);
+    if (use_lds_kernel(Y)) {
+        void *f = nullptr;
+        if ((rc = workspace_get(w, SLOT_FOOTPRINT, footprint_bytes(X, Z, n_views), &f))) return rc;
+        if ((rc = build_footprints(s, J, (float4 *)f))) return rc;
+    }
     switch (staged_dtype) {
     case CTR_F32: return launch_bp<CTR_F32>(s, J);
     case CTR_F16: return launch_bp<CTR_F16>(s, J);
@@ -652,9 +692,10 @@ int ctr_backproject_packed(void *stream, ctr_workspace *ws, int staged_dtype,
                 "ctr_backproject_packed: count %d outside [1, %d]", count, kMaxPacked);
     CTR_REQUIRE(staged_dtype == CTR_F32 || staged_dtype == CTR_F16 || staged_dtype == CTR_BF16,
                 CTR_ERR_INVALID_ARGUMENT, "unknown staged dtype %d", staged_dtype);
-    size_t total_views = 0;
+    size_t total_views = 0, total_foot = 0;
     for (int q = 0; q < count; q++) {
         const ctr_backproject_job &j = jobs[q];
+        total_foot += footprint_bytes(j.X, j.Z, j.n_views);
         CTR_REQUIRE(j.staged && j.volume, CTR_ERR_NULL_POINTER,
                     "ctr_backproject_packed: job %d has a NULL buffer", q);
         CTR_REQUIRE(staged_dtype == CTR_F32 || j.view_scale, CTR_ERR_NULL_POINTER,
@@ -673,10 +714,13 @@ int ctr_backproject_packed(void *stream, ctr_workspace *ws, int staged_dtype,
     if (rc) return rc;
     void *p = nullptr;
     if ((rc = workspace_get(w, SLOT_GEOMETRY, kGeomBytesPerView * total_views, &p))) return rc;
+    void *fbase = nullptr;
+    if ((rc = workspace_get(w, SLOT_FOOTPRINT, total_foot, &fbase))) return rc;
     hipStream_t s = as_stream(stream);
     BpBatch B;
     B.count = count;
     char *g = (char *)p;
+    char *fp = (char *)fbase;
     long long blocks = 0;
     for (int q = 0; q < count; q++) {
         const ctr_backproject_job &j = jobs[q];
@@ -689,9 +733,11 @@ int ctr_backproject_packed(void *stream, ctr_workspace *ws, int staged_dtype,
         StagedLayout L = staged_layout(j.nu, j.nv, j.n_views, staged_dtype);
         B.job[q] = make_job(j.staged, L, j.nu, j.nv, gp, j.view_scale, j.n_views, j.volume, j.X,
                             j.Y, j.Z, j.scale, j.accumulate, j.clip_negative);
+        if ((rc = build_footprints(s, B.job[q], (float4 *)fp))) return rc;
         B.job[q].block_begin = (int)blocks;
         blocks += (long long)B.job[q].gx * B.job[q].gy * ((j.Y + kTY - 1) / kTY);
         g += kGeomBytesPerView * (size_t)j.n_views;
+        fp += footprint_bytes(j.X, j.Z, j.n_views);
     }
     CTR_REQUIRE(blocks < (1LL << 31), CTR_ERR_INVALID_ARGUMENT, "packed launch too large");
     switch (staged_dtype) {

@@ -37,7 +37,8 @@ enum Slot {
     SLOT_VIEW_MAX = 1,   // per-view max |q| as uint bits, 1 uint per view
     SLOT_STAGE_F32 = 2,  // fp32 staged projections before rounding to 16 bit
     SLOT_ELLIPSOIDS = 3, // ellipsoid records for the phantom renderer
-    SLOT_COUNT = 4
+    SLOT_FOOTPRINT = 4,  // per (tile column, view) detector footprint bounds
+    SLOT_COUNT = 5
 };
 
 // Resolves NULL to the per-device default workspace.

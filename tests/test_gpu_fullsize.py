@@ -113,9 +113,12 @@ def test_full_size_filter_rows_sample(ctr, orc, dev):
 
 
 def test_backproject_kernel_variants_agree(ctr, dev, monkeypatch):
-    """The LDS-staged kernel and the direct-tap kernel give bit-identical
-    volumes: both take their coordinates from the same FP64 column set-up and
-    the LDS footprint holds exactly the taps the direct kernel reads."""
+    """The LDS-staged kernel and the direct-tap kernel agree to FP32
+    rounding: both take their coordinates from the same FP64 column set-up
+    and the LDS footprint holds exactly the taps the direct kernel reads; only
+    the y-run length (32 vs 16 voxels) changes where the FP32 row step
+    starts; the gap (about 2e-6 of the peak) is FP32 rounding of that step,
+    fifty times inside the 1e-4 parity gate."""
     import torch
     g = ctr.config_geometry("fdk_256_s")
     q = _random_views(ctr, g, dev, 5)
@@ -126,7 +129,7 @@ def test_backproject_kernel_variants_agree(ctr, dev, monkeypatch):
         monkeypatch.setenv("CTR_BACKPROJECT_KERNEL", "gather")
         b = ctr.backproject(staged, g, dt, view_scale=vs, scale=0.01)
         torch.cuda.synchronize()
-        assert torch.equal(a, b)
+        assert rel_max(a.cpu().numpy(), b.cpu().numpy()) < 5e-6
 
 
 def test_lds_kernel_falls_back_on_wide_footprints(ctr, orc, dev, monkeypatch):
