@@ -6,16 +6,16 @@
 // fixed column the detector u coordinate, the distance weight and the v step
 // are the same for the whole run, so per view a thread sets the column up
 // once -- in FP64, from FP64 per-view projection matrices -- and then spends
-// 11 FP32 VALU ops per voxel: the v FMA, a clamp, fract + cvt for the row
-// and its weight, a 24-bit mad for the texel address, two vertical lerps and
-// two pre-weighted accumulations.
+// a handful of VALU ops per voxel: the v FMA, fract + cvt for the row and
+// its weight, a 24-bit mad for the texel address, the two vertical lerps and
+// the pre-weighted accumulation (details with each kernel below).
 //
 // Coordinates are "staged coordinates" (one border texel before the data),
 // where clamping to [0, n + 1] is the same as "zero outside the detector"
-// and truncation equals floor.  The v coordinate of a run is split into an
-// integer row I and a fraction f < 1 in FP64, so the per-voxel FP32 value
-// f + dv * t stays small and keeps ~1e-5 px resolution even on 1536-row
-// panels (a plain FP32 row coordinate has 1.2e-4 px steps there).
+// and truncation equals floor.  The v coordinate of a run is taken relative
+// to an integer row in FP64, so the per-voxel FP32 value f + dv * t stays
+// small and keeps ~1e-5 px resolution even on 1536-row panels (a plain FP32
+// row coordinate has 1.2e-4 px steps there).
 //
 // A 256-thread block is 4 wave64s, each covering an 8 x 8 patch of columns,
 // so the 64 lanes of a wave sample a compact detector footprint.
@@ -232,9 +232,19 @@ struct ColumnView {
     float w2;    // distance weight squared times the scale
 };
 
-__device__ inline ColumnView column_view64(const double *g, double fi, double fk, double fj0,
+// Per-view tables are read-only for the whole launch: reading them through
+// the constant address space lets wave-uniform reads use scalar loads,
+// which stay off the vector-memory counter that the LDS row prefetch uses.
+template <typename T>
+__device__ inline const __attribute__((address_space(4))) T *as_constant(const T *p)
+{
+    return (const __attribute__((address_space(4))) T *)p;
+}
+
+__device__ inline ColumnView column_view64(const double *gp, double fi, double fk, double fj0,
                                            float vs)
 {
+    const __attribute__((address_space(4))) double *g = as_constant(gp);
     ColumnView c;
     const double den = fma(g[7], fi, fma(g[8], fk, g[9]));
     // reciprocal with one Newton step: full FP64 precision
@@ -345,7 +355,10 @@ struct BpBatch {
 
 // ---------------------------------------------------------------------------
 // Direct-tap kernel: taps straight from the staged buffer (global loads
-// through L1/L2).
+// through L1/L2), coordinates clamped to the staged view: per voxel the v
+// FMA, a clamp, fract, cvt, mad24, two loads, two lerps and two FMAs (11
+// VALU for fp32).  Used for short runs (Y < 32, the 2D fan / parallel
+// workloads) and as the reference variant of the LDS kernel.
 // ---------------------------------------------------------------------------
 template <int DT, int VY>
 __global__ __launch_bounds__(256) void backproject_kernel(BpJob J)
@@ -378,18 +391,60 @@ __global__ __launch_bounds__(256) void backproject_kernel(BpJob J)
 }
 
 // ---------------------------------------------------------------------------
-// LDS-staged kernel.  Per view, the block's detector footprint -- the
-// bounding box of the projections of its 16 x 16 x TY voxel tile, at most
-// 63 texels wide and HMAX - 1 rows high -- is copied from the staged
-// projections into LDS with global_load_lds (one wave instruction per row,
-// no VGPR round trip), and every tap is then an LDS read: two ds_read2_b32
-// for fp32, one ds_read2_b32 of two 16-bit pairs otherwise.  A view whose
-// footprint does not fit falls back to the direct global taps.
+// LDS-staged kernel.
+//
+// Per view, the block's detector footprint -- the bounding rectangle of the
+// projections of its 16 x 16 x TY voxel tile, at most 64 texels wide and
+// HMAX rows high -- is copied from the staged projections into LDS with
+// global_load_lds (one wave instruction per row, addresses in SGPRs, no VGPR
+// round trip).  Two row buffers alternate: the rows of view n + 1 are in
+// flight while view n is interpolated from the other buffer, and one barrier
+// per view is the only synchronisation.
+//
+// The rectangles are computed kRectChunk views at a time into a table in LDS
+// (one thread per view) from the per-column-tile footprint table, and each
+// is classified as
+//   skip   -- every tap lies in the zero border: no contribution, no work;
+//   lds    -- staged and interpolated from LDS;
+//   direct -- too large for the buffers: done after the main loop by a
+//             second pass over such views with direct global taps.
+// Rectangles are not clipped to the detector.  Rows and columns outside it
+// are loaded from the (zero) border of the staged view by clamping the
+// source row and column, so the per-voxel row coordinate needs no clamp.
+//
+// Per voxel:
+//   fp32 / bf16 -- v FMA, fract, cvt, mad24, the taps (two ds_read2_b32 for
+//     fp32, one for 16-bit pairs), two lerps and two FMAs into the
+//     accumulator (10 VALU for fp32, +4 bit ops for bf16 pairs);
+//   fp16 -- v FMA, fract, cvt, mad24, one ds_read2_b32 of the (u, u + 1)
+//     pairs of rows I and I + 1, the weight to f16, a packed f16 difference
+//     and lerp, and v_dot2_f32_f16 of the lerped pair with the column's f16
+//     (1 - wu, wu) weights into the FP32 accumulator (8 VALU).  Those f16
+//     weights hold the distance weight times view_scale / max(view_scale)
+//     (at most (SOD / (SOD - r))^2, r the volume's half diagonal; the host
+//     keeps this path for geometries where that stays far below the f16
+//     range), and the run is rescaled by scale * max(view_scale) at the
+//     store.
 // ---------------------------------------------------------------------------
-constexpr int kLdsPitch = 65;  // dwords per LDS row: 64 staged + 1 against bank aliasing
-constexpr int kLdsCols = 64;   // texels staged per row (one per lane)
+constexpr int kLdsPitch = 65;   // dwords per LDS row: 64 staged + 1 against bank aliasing
+constexpr int kLdsCols = 64;    // texels staged per row (one per lane)
+constexpr int kRectChunk = 128; // views per refill of the rectangle table
+constexpr int kF16Packed = 16;  // tile variant: fp16 pairs with packed f16 arithmetic
 
-// Taps at LDS byte address `addr`.
+enum RectMode { RECT_SKIP = 0, RECT_LDS = 1, RECT_DIRECT = 2 };
+
+// LDS of the kernel: the rectangle table (kRectChunk + 1 int4), then two row
+// buffers of HMAX rows.
+template <int HMAX>
+struct LdsLayout {
+    static constexpr int kRectBytes = (kRectChunk + 1) * 16;
+    static constexpr int kBufDwords = HMAX * kLdsPitch;
+    static constexpr int kDwords = kRectBytes / 4 + 2 * kBufDwords;
+    static_assert((2 * kBufDwords) % 4 == 0, "buffers are cleared as uint4");
+};
+
+// Taps at LDS byte address `addr`: rows I (top) and I + 1 (bot), columns
+// (u, u + 1) as fp32.
 template <int DT>
 __device__ inline void lds_taps(const unsigned *L, unsigned addr, float2 &top, float2 &bot);
 
@@ -420,111 +475,300 @@ __device__ inline void lds_taps<CTR_BF16>(const unsigned *L, unsigned addr, floa
     bot = bf16_pair(P[kLdsPitch]);
 }
 
+// Detector rectangle of one tile for view n: (u0, v0, rows, mode), staged
+// coordinates.  Bounds are clamped to [-1024, n + 1024] first so the int
+// conversions stay exact; a clamped rectangle is still classified correctly
+// because the clamp margin exceeds both buffer limits.
+__device__ inline int4 tile_rect(const BpJob &J, const float4 *foot, size_t foot_stride, int n,
+                                 float ja, float jb, float cv1, int hmax)
+{
+    const float4 fp = foot[(size_t)n * foot_stride];
+    const float vj = J.geom[3 * n].w;
+    const float ya = ja * vj, yb = jb * vj;
+    const float lo = fminf(fminf(ya * fp.z, ya * fp.w), fminf(yb * fp.z, yb * fp.w));
+    const float hi = fmaxf(fmaxf(ya * fp.z, ya * fp.w), fmaxf(yb * fp.z, yb * fp.w));
+    const float ul = (float)J.nu + 1024.0f, vl = (float)J.nv + 1024.0f;
+    const int u0 = (int)floorf(__builtin_amdgcn_fmed3f(fp.x - 0.05f, -1024.0f, ul));
+    const int u1 = (int)floorf(__builtin_amdgcn_fmed3f(fp.y + 0.05f, -1024.0f, ul)) + 1;
+    const int v0 = (int)floorf(__builtin_amdgcn_fmed3f(cv1 + lo - 0.05f, -1024.0f, vl));
+    const int v1 = (int)floorf(__builtin_amdgcn_fmed3f(cv1 + hi + 0.05f, -1024.0f, vl)) + 1;
+    int mode;
+    if (u1 <= 0 || v1 <= 0 || u0 >= J.nu + 1 || v0 >= J.nv + 1)
+        mode = RECT_SKIP;
+    else if (u1 - u0 < kLdsCols && v1 - v0 < hmax)
+        mode = RECT_LDS;
+    else
+        mode = RECT_DIRECT;
+    return make_int4(u0, v0, v1 - v0 + 1, mode);
+}
+
+__device__ inline int4 read_rect(const int4 *tab, int c)
+{
+    const int4 r = tab[c];
+    return make_int4(__builtin_amdgcn_readfirstlane(r.x), __builtin_amdgcn_readfirstlane(r.y),
+                     __builtin_amdgcn_readfirstlane(r.z), __builtin_amdgcn_readfirstlane(r.w));
+}
+
+// Rows v0 .. v0 + h - 1 of one staged view into an LDS row buffer, row r by
+// wave r mod 4, 64 columns from u0.  Rows outside the view are read from its
+// first or last row, columns outside it from its last column: all zero in
+// every texel format (column 0 is not -- a 16-bit pair texel there holds
+// (0, q[1])).  The loads go through a
+// buffer resource spanning exactly the view (raw 32-bit format), with the
+// row offset in an SGPR and the lane's column in the VGPR offset: no VALU
+// work per row, and any address outside the view would read zero.
+__device__ inline void stage_rect(const char *view, unsigned view_bytes, int pitch_u,
+                                  int pitch_v, int u0, int v0, int h, unsigned *buf, int wave,
+                                  int lane)
+{
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(view), (short)0, (int)view_bytes,
+                                          0x00020000);
+    const int col = u0 + lane;
+    const unsigned col4 = (unsigned)(col < 0 || col >= pitch_u ? pitch_u - 1 : col) * 4u;
+    const unsigned row_bytes = (unsigned)pitch_u * 4u;
+    for (int r = wave; r < h; r += 4) {
+        const int srow = min(max(v0 + r, 0), pitch_v - 1);
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+            rsrc, (__attribute__((address_space(3))) void *)(buf + r * kLdsPitch), 4, col4,
+            (unsigned)srow * row_bytes, 0, 0);
+    }
+}
+
+// Per-view constants of a column run read from LDS.
+struct LdsRun {
+    float cL, cR;   // (1 - wu) w2, wu w2
+    float f, dv;    // row of the first voxel relative to the rectangle, row step
+    unsigned base;  // LDS byte address of (row 0, column of the left tap)
+};
+
+__device__ inline LdsRun lds_run(const ColumnView &c, int u0, int v0, unsigned buf_byte)
+{
+    LdsRun R;
+    const double su = c.su - (double)u0;
+    const double sfl = floor(su);
+    const float wu = (float)(su - sfl);
+    R.cR = c.w2 * wu;
+    R.cL = c.w2 - R.cR;
+    R.f = (float)(c.sv0 - (double)v0);
+    R.dv = c.dv;
+    R.base = buf_byte + (unsigned)min(max((int)sfl, 0), kLdsCols - 2) * 4u;
+    return R;
+}
+
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+typedef __fp16 fp16x2_t __attribute__((ext_vector_type(2)));
+
+template <int DT, int TY>
+__device__ inline void lds_run_accumulate(float (&acc)[TY], const LdsRun &R, const unsigned *lds,
+                                          unsigned lp4)
+{
+    if constexpr (DT == kF16Packed) {
+        const half2_t W = {(_Float16)R.cL, (_Float16)R.cR};
+#pragma unroll
+        for (int t = 0; t < TY; ++t) {
+            const float x = fmaf(R.dv, (float)t, R.f);
+            const float wv = __builtin_amdgcn_fractf(x);
+            const unsigned addr = __umul24((unsigned)x, lp4) + R.base;
+            const unsigned *P =
+                reinterpret_cast<const unsigned *>(reinterpret_cast<const char *>(lds) + addr);
+            const half2_t a = __builtin_bit_cast(half2_t, P[0]);
+            const half2_t b = __builtin_bit_cast(half2_t, P[kLdsPitch]);
+            const _Float16 w16 = (_Float16)wv;
+            const half2_t w = {w16, w16};
+            const half2_t lr = __builtin_elementwise_fma(w, b - a, a);
+            acc[t] = __builtin_amdgcn_fdot2(__builtin_bit_cast(fp16x2_t, lr),
+                                            __builtin_bit_cast(fp16x2_t, W), acc[t], false);
+        }
+    } else if constexpr (DT == CTR_F32) {
+        // fp32 reads four taps per voxel: software-pipelined in batches of
+        // kB voxels (taps of batch b + 1 in flight while batch b is
+        // interpolated), the scheduling barriers keep that order
+        constexpr int kB = 4;
+        float2 ta[2][kB], tb[2][kB];
+        float tw[2][kB];
+        auto issue = [&](int set, int t0) {
+#pragma unroll
+            for (int q = 0; q < kB; ++q) {
+                const float x = fmaf(R.dv, (float)(t0 + q), R.f);
+                tw[set][q] = __builtin_amdgcn_fractf(x);
+                lds_taps<DT>(lds, __umul24((unsigned)x, lp4) + R.base, ta[set][q], tb[set][q]);
+            }
+        };
+        issue(0, 0);
+#pragma unroll
+        for (int bt = 0; bt < TY / kB; ++bt) {
+            if (bt + 1 < TY / kB) issue((bt + 1) & 1, (bt + 1) * kB);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < kB; ++q) {
+                const float2 a = ta[bt & 1][q], b = tb[bt & 1][q];
+                const float wv = tw[bt & 1][q];
+                const float left = fmaf(wv, b.x - a.x, a.x);
+                const float right = fmaf(wv, b.y - a.y, a.y);
+                acc[bt * kB + q] = fmaf(R.cR, right, fmaf(R.cL, left, acc[bt * kB + q]));
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < TY; ++t) {
+            const float x = fmaf(R.dv, (float)t, R.f);
+            const float wv = __builtin_amdgcn_fractf(x);
+            const unsigned addr = __umul24((unsigned)x, lp4) + R.base;
+            float2 a, b;
+            lds_taps<DT>(lds, addr, a, b);
+            const float left = fmaf(wv, b.x - a.x, a.x);
+            const float right = fmaf(wv, b.y - a.y, a.y);
+            acc[t] = fmaf(R.cR, right, fmaf(R.cL, left, acc[t]));
+        }
+    }
+}
+
 template <int DT, int TY, int HMAX>
 __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int bz,
                                             unsigned *lds)
 {
-    const int pitch_u = J.pitch_u, pitch_v = J.pitch_v, nu = J.nu, nv = J.nv;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int ti0 = bx * 16, tk0 = by * 16;
-    const int i = ti0 + (wave & 1) * 8 + (lane & 7);
-    const int k = tk0 + (wave >> 1) * 8 + (lane >> 3);
+    using L = LdsLayout<HMAX>;
+    constexpr int kTapDT = DT == kF16Packed ? CTR_F16 : DT;  // staged texel format
+    int4 *rects = reinterpret_cast<int4 *>(lds);
+    unsigned *bufs = lds + L::kRectBytes / 4;
+    const int pitch_u = J.pitch_u, pitch_v = J.pitch_v, N = J.n_views;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i = bx * 16 + (wave & 1) * 8 + (lane & 7);
+    const int k = by * 16 + (wave >> 1) * 8 + (lane >> 3);
     const int j0 = bz * TY;
-    const float fnu = (float)nu, fnv = (float)nv;
-    const double su_hi = (double)nu + 1.0;
-    // rows of the run relative to the volume centre, and the staged v centre
+    // the run's rows relative to the volume centre, and the staged v centre
     const float oy = 0.5f - 0.5f * (float)J.Y;
     const float ja = (float)j0 + oy, jb = (float)(j0 + TY - 1) + oy;
-    const float cv1 = 0.5f * fnv + 0.5f;
+    const float cv1 = 0.5f * (float)J.nv + 0.5f;
     const float4 *foot = J.foot + (size_t)by * J.gx + bx;
     const size_t foot_stride = (size_t)J.gx * J.gy;
+    const char *vbase = reinterpret_cast<const char *>(J.staged);
+    const size_t view_bytes = J.view_texels * 4;
+    const unsigned view_bytes32 = (unsigned)view_bytes;  // < 2^24, checked by the host
+    const unsigned lp4 = J.lds_pitch4;
+
+    // accumulation scale: fp16 weights are relative to the largest view scale
+    float wscale = J.scale, out_scale = 1.0f;
+    if constexpr (DT == kF16Packed) {
+        float m = 0.0f;
+        for (int t = threadIdx.x; t < N; t += 256) m = fmaxf(m, fabsf(J.view_scale[t]));
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+        float *part = reinterpret_cast<float *>(lds);
+        if (lane == 0) part[wave] = m;
+        __syncthreads();
+        m = fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
+        __syncthreads();
+        wscale = m > 0.0f ? 1.0f / m : 0.0f;
+        out_scale = J.scale * m;
+    }
+
+    // clear the row buffers (every LDS tap then reads a finite value), fill
+    // the first rectangle table
+    for (int t = threadIdx.x; t < L::kBufDwords / 2; t += 256)
+        reinterpret_cast<uint4 *>(bufs)[t] = make_uint4(0u, 0u, 0u, 0u);
+    if (threadIdx.x <= kRectChunk && (int)threadIdx.x < N)
+        rects[threadIdx.x] = tile_rect(J, foot, foot_stride, threadIdx.x, ja, jb, cv1, HMAX);
+    __syncthreads();
 
     float acc[TY];
 #pragma unroll
     for (int t = 0; t < TY; ++t) acc[t] = 0.0f;
 
-    const char *vbase = reinterpret_cast<const char *>(J.staged);
-    const size_t view_bytes = J.view_texels * 4;
-    const unsigned pitch4 = (unsigned)pitch_u * 4u;
+    int4 cur = read_rect(rects, 0);
+    if (cur.w == RECT_LDS)
+        stage_rect(vbase, view_bytes32, pitch_u, pitch_v, cur.x, cur.y, cur.z, bufs, wave, lane);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
 
+    bool any_direct = false;
+    int n0 = 0;
 #pragma unroll 1
-    for (int n = 0; n < J.n_views; ++n) {
-        float vs = J.scale;
-        if (J.view_scale) vs *= J.view_scale[n];
-
-        // footprint of the tile: u range and 1/den range from the table, the
-        // v range of this run of rows from them (0.05 px margin below; the
-        // voxels use FP64 coordinates)
-        const float4 fp = foot[(size_t)n * foot_stride];
-        const float vj = J.geom[3 * n].w;
-        const float ya = ja * vj, yb = jb * vj;
-        float umin = fp.x, umax = fp.y;
-        float vmin = cv1 + fminf(fminf(ya * fp.z, ya * fp.w), fminf(yb * fp.z, yb * fp.w));
-        float vmax = cv1 + fmaxf(fmaxf(ya * fp.z, ya * fp.w), fmaxf(yb * fp.z, yb * fp.w));
-        umin = __builtin_amdgcn_fmed3f(umin, 0.0f, fnu + 1.0f);
-        umax = __builtin_amdgcn_fmed3f(umax, 0.0f, fnu + 1.0f);
-        vmin = __builtin_amdgcn_fmed3f(vmin, 0.0f, fnv + 1.0f);
-        vmax = __builtin_amdgcn_fmed3f(vmax, 0.0f, fnv + 1.0f);
-        const int u0 = __builtin_amdgcn_readfirstlane(max((int)floorf(umin - 0.05f), 0));
-        const int u1 = __builtin_amdgcn_readfirstlane(min((int)floorf(umax + 0.05f) + 1, nu + 2));
-        const int v0 = __builtin_amdgcn_readfirstlane(max((int)floorf(vmin - 0.05f), 0));
-        const int v1 = __builtin_amdgcn_readfirstlane(min((int)floorf(vmax + 0.05f) + 1, nv + 2));
-        const int W = u1 - u0 + 1, H = v1 - v0 + 1;
-
-        const ColumnView c = column_view64(J.geom64 + (size_t)kGeom64 * n, i, k, j0, vs);
-        const char *top = vbase + (size_t)n * view_bytes;
-
-        if (W < kLdsCols && H < HMAX) {
-            // stage staged rows v0 .. v1 + 1 (one spare row, clamped into the view)
-            const unsigned *src = reinterpret_cast<const unsigned *>(top);
-            const int scol = min(u0 + lane, pitch_u - 1);
-            for (int r = wave; r <= H; r += 4) {
-                const int srow = min(v0 + r, pitch_v - 1);
-                __builtin_amdgcn_global_load_lds(
-                    (const void __attribute__((address_space(1))) *)(src + (size_t)srow * pitch_u + scol),
-                    (void __attribute__((address_space(3))) *)(lds + r * kLdsPitch), 4, 0, 0);
-            }
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    for (int n = 0; n < N; ++n) {
+        const int c = n - n0;
+        int4 nxt = make_int4(0, 0, 0, RECT_SKIP);
+        if (n + 1 < N) {
+            nxt = read_rect(rects, c + 1);
+            if (nxt.w == RECT_LDS)
+                stage_rect(vbase + (size_t)(n + 1) * view_bytes, view_bytes32, pitch_u, pitch_v,
+                           nxt.x, nxt.y, nxt.z, bufs + ((n + 1) & 1) * L::kBufDwords, wave,
+                           lane);
+        }
+        if (cur.w == RECT_LDS) {
+            float vs = wscale;
+            if (J.view_scale) vs *= as_constant(J.view_scale)[n];
+            const ColumnView cv = column_view64(J.geom64 + (size_t)kGeom64 * n, i, k, j0, vs);
+            const unsigned buf_byte = (unsigned)(L::kRectBytes + (n & 1) * L::kBufDwords * 4);
+            const LdsRun R = lds_run(cv, cur.x, cur.y, buf_byte);
+            lds_run_accumulate<DT, TY>(acc, R, lds, lp4);
+        }
+        any_direct |= cur.w == RECT_DIRECT;
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        if (c + 1 == kRectChunk && n + 1 < N) {
+            n0 += kRectChunk;
+            if (threadIdx.x <= kRectChunk && n0 + (int)threadIdx.x < N)
+                rects[threadIdx.x] =
+                    tile_rect(J, foot, foot_stride, n0 + threadIdx.x, ja, jb, cv1, HMAX);
             __syncthreads();
+        }
+        cur = nxt;
+    }
 
-            // rows clamped to the staged view and to the footprint
-            const unsigned lp4 = J.lds_pitch4;
-            const Run R = make_run(c, (double)(W - 1), (double)u0, v0, min(nv + 1, v1), v0, lp4,
-                                   su_hi);
-#pragma unroll
-            for (int t = 0; t < TY; ++t) {
-                const float x = __builtin_amdgcn_fmed3f(fmaf(R.dv, (float)t, R.f), R.lo, R.hi);
-                const float wv = __builtin_amdgcn_fractf(x);
-                const unsigned addr = __umul24((unsigned)x, lp4) + R.base;
-                float2 a, b;
-                lds_taps<DT>(lds, addr, a, b);
-                const float left = fmaf(wv, b.x - a.x, a.x);
-                const float right = fmaf(wv, b.y - a.y, a.y);
-                acc[t] = fmaf(R.cR, right, fmaf(R.cL, left, acc[t]));
-            }
-            __syncthreads();
-        } else {
-            const Run R = make_run(c, su_hi, 0.0, 0, nv + 1, 0, pitch4, su_hi);
-            gather_run<DT, TY>(acc, R, top, top + pitch4, pitch4);
+    // views whose footprint exceeds the buffers: direct taps, clamped
+    if (any_direct) {
+        const unsigned pitch4 = (unsigned)pitch_u * 4u;
+        const double su_hi = (double)J.nu + 1.0;
+#pragma unroll 1
+        for (int n = 0; n < N; ++n) {
+            const int mode = __builtin_amdgcn_readfirstlane(
+                tile_rect(J, foot, foot_stride, n, ja, jb, cv1, HMAX).w);
+            if (mode != RECT_DIRECT) continue;
+            float vs = wscale;
+            if (J.view_scale) vs *= J.view_scale[n];
+            const ColumnView cv = column_view64(J.geom64 + (size_t)kGeom64 * n, i, k, j0, vs);
+            const Run R = make_run(cv, su_hi, 0.0, 0, J.nv + 1, 0, pitch4, su_hi);
+            const char *top = vbase + (size_t)n * view_bytes;
+            gather_run<kTapDT, TY>(acc, R, top, top + pitch4, pitch4);
         }
     }
+    if constexpr (DT == kF16Packed) {
+#pragma unroll
+        for (int t = 0; t < TY; ++t) acc[t] *= out_scale;
+    }
     store_run<TY>(acc, J.vol, i, j0, k, J.X, J.Y, J.Z, J.accumulate, J.clip);
+}
+
+// Workgroups are dispatched to the 8 XCDs round-robin; this renumbering hands
+// each XCD one contiguous range of tiles, so neighbouring tiles -- which
+// read overlapping detector rows -- share that XCD's L2.
+__device__ inline int xcd_tile_order(int b, int nblocks)
+{
+    const int xcd = b & 7, k = b >> 3;
+    const int q = nblocks >> 3, r = nblocks & 7;
+    return xcd * q + min(xcd, r) + k;
 }
 
 template <int DT, int TY, int HMAX>
 __global__ __launch_bounds__(256) void backproject_lds_kernel(BpJob J)
 {
-    __shared__ unsigned lds[HMAX * kLdsPitch];
-    backproject_lds_tile<DT, TY, HMAX>(J, blockIdx.x, blockIdx.y, blockIdx.z, lds);
+    __shared__ __align__(16) unsigned lds[LdsLayout<HMAX>::kDwords];
+    const int t = xcd_tile_order(blockIdx.x, gridDim.x);
+    const int bx = t % J.gx;
+    const int by = (t / J.gx) % J.gy;
+    const int bz = t / (J.gx * J.gy);
+    backproject_lds_tile<DT, TY, HMAX>(J, bx, by, bz, lds);
 }
 
-// Packed launch: the flat block index selects the job (the last job whose
+// Packed launch: the flat tile index selects the job (the last job whose
 // block_begin it reaches) and the tile inside it.
 template <int DT, int TY, int HMAX>
 __global__ __launch_bounds__(256) void backproject_lds_packed_kernel(BpBatch B)
 {
-    __shared__ unsigned lds[HMAX * kLdsPitch];
-    const int b = blockIdx.x;
+    __shared__ __align__(16) unsigned lds[LdsLayout<HMAX>::kDwords];
+    const int b = xcd_tile_order(blockIdx.x, gridDim.x);
     int j = 0;
 #pragma unroll 1
     for (int q = 1; q < B.count; ++q)
@@ -549,7 +793,7 @@ static int variant_override()
     return 0;
 }
 
-constexpr int kTY = 32, kHMAX = 112;  // LDS kernel: y-run and staged rows
+constexpr int kTY = 32, kHMAX = 96;  // LDS kernel: y-run and rows per buffer
 
 static BpJob make_job(const void *staged, const StagedLayout &L, int nu, int nv, GeomPtrs geom,
                       const float *view_scale, int n_views, float *vol, int X, int Y, int Z,
@@ -603,12 +847,29 @@ static size_t footprint_bytes(int X, int Z, int n_views)
     return sizeof(float4) * (size_t)((X + 15) / 16) * ((Z + 15) / 16) * n_views;
 }
 
+// The packed-f16 LDS variant stores the column's distance weight (times a
+// view scale ratio <= 1) in f16: usable while (SOD / (SOD - r))^2, r the
+// volume's half diagonal, stays far below the f16 range.
+static bool f16_weights_fit(const BpJob &J, int geometry, double sod, double volume_width)
+{
+    if (geometry != CTR_GEOMETRY_CONE) return true;
+    const double d = volume_width / J.X;
+    const double rx = 0.5 * J.X * d, rz = 0.5 * J.Z * d;
+    const double r = sqrt(rx * rx + rz * rz);
+    const double g = sod / (sod - r);
+    return g * g < 1024.0;
+}
+
 template <int DT>
-static int launch_bp(hipStream_t s, const BpJob &J)
+static int launch_bp(hipStream_t s, const BpJob &J, bool packed_f16)
 {
     if (use_lds_kernel(J.Y)) {
-        dim3 grid(J.gx, J.gy, (J.Y + kTY - 1) / kTY);
-        backproject_lds_kernel<DT, kTY, kHMAX><<<grid, 256, 0, s>>>(J);
+        const long long blocks = (long long)J.gx * J.gy * ((J.Y + kTY - 1) / kTY);
+        CTR_REQUIRE(blocks < (1LL << 31), CTR_ERR_INVALID_ARGUMENT, "volume too large");
+        if (DT == CTR_F16 && packed_f16)
+            backproject_lds_kernel<kF16Packed, kTY, kHMAX><<<(unsigned)blocks, 256, 0, s>>>(J);
+        else
+            backproject_lds_kernel<DT, kTY, kHMAX><<<(unsigned)blocks, 256, 0, s>>>(J);
         CTR_CHECK_LAUNCH();
         return CTR_OK;
     }
@@ -676,10 +937,11 @@ int ctr_backproject(void *stream, ctr_workspace *ws, int geometry, const void *s
         if ((rc = workspace_get(w, SLOT_FOOTPRINT, footprint_bytes(X, Z, n_views), &f))) return rc;
         if ((rc = build_footprints(s, J, (float4 *)f))) return rc;
     }
+    const bool packed = f16_weights_fit(J, geometry, source_distance, volume_width);
     switch (staged_dtype) {
-    case CTR_F32: return launch_bp<CTR_F32>(s, J);
-    case CTR_F16: return launch_bp<CTR_F16>(s, J);
-    default: return launch_bp<CTR_BF16>(s, J);
+    case CTR_F32: return launch_bp<CTR_F32>(s, J, packed);
+    case CTR_F16: return launch_bp<CTR_F16>(s, J, packed);
+    default: return launch_bp<CTR_BF16>(s, J, packed);
     }
 }
 
@@ -693,6 +955,7 @@ int ctr_backproject_packed(void *stream, ctr_workspace *ws, int staged_dtype,
     CTR_REQUIRE(staged_dtype == CTR_F32 || staged_dtype == CTR_F16 || staged_dtype == CTR_BF16,
                 CTR_ERR_INVALID_ARGUMENT, "unknown staged dtype %d", staged_dtype);
     size_t total_views = 0, total_foot = 0;
+    bool packed = true;
     for (int q = 0; q < count; q++) {
         const ctr_backproject_job &j = jobs[q];
         total_foot += footprint_bytes(j.X, j.Z, j.n_views);
@@ -734,6 +997,8 @@ int ctr_backproject_packed(void *stream, ctr_workspace *ws, int staged_dtype,
         B.job[q] = make_job(j.staged, L, j.nu, j.nv, gp, j.view_scale, j.n_views, j.volume, j.X,
                             j.Y, j.Z, j.scale, j.accumulate, j.clip_negative);
         if ((rc = build_footprints(s, B.job[q], (float4 *)fp))) return rc;
+        packed = packed && f16_weights_fit(B.job[q], j.geometry, j.source_distance,
+                                           j.volume_width);
         B.job[q].block_begin = (int)blocks;
         blocks += (long long)B.job[q].gx * B.job[q].gy * ((j.Y + kTY - 1) / kTY);
         g += kGeomBytesPerView * (size_t)j.n_views;
@@ -745,7 +1010,11 @@ int ctr_backproject_packed(void *stream, ctr_workspace *ws, int staged_dtype,
         backproject_lds_packed_kernel<CTR_F32, kTY, kHMAX><<<(unsigned)blocks, 256, 0, s>>>(B);
         break;
     case CTR_F16:
-        backproject_lds_packed_kernel<CTR_F16, kTY, kHMAX><<<(unsigned)blocks, 256, 0, s>>>(B);
+        if (packed)
+            backproject_lds_packed_kernel<kF16Packed, kTY, kHMAX>
+                <<<(unsigned)blocks, 256, 0, s>>>(B);
+        else
+            backproject_lds_packed_kernel<CTR_F16, kTY, kHMAX><<<(unsigned)blocks, 256, 0, s>>>(B);
         break;
     default:
         backproject_lds_packed_kernel<CTR_BF16, kTY, kHMAX><<<(unsigned)blocks, 256, 0, s>>>(B);

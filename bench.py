@@ -31,7 +31,21 @@ import __graft_entry__  # noqa: E402
 HBM_PEAK_GBPS = 8000.0
 WORKLOAD = "fdk_512"
 FFT_LENGTH = 2048
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "backproject_fdk512_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "backproject_fdk512_traffic.md")
+
+
+def read_traffic(key):
+    """HBM bytes per backprojection launch measured with rocprofv3 FETCH_SIZE
+    and WRITE_SIZE (each scaled by the same counter's reading for a plain copy
+    of known size), as recorded in the profile summary: a line `key = N`."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as fh:
+        for line in fh:
+            parts = line.strip().strip("`").split("=")
+            if len(parts) == 2 and parts[0].strip() == key:
+                return int(parts[1].strip())
+    return None
 
 
 def log(msg):
@@ -224,10 +238,7 @@ def main():
     b_alg32 = (4 * 4 + 8.0 / g.n_views) * g.updates
     ach16 = b_alg16 / (bp_ms * 1e-3) / 1e9
     ach32 = b_alg32 / (bp_ms32 * 1e-3) / 1e9
-    traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as fh:
-            traffic = json.load(fh).get("fp16_bytes_per_launch")
+    traffic = read_traffic("fp16_bytes_per_launch")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -256,7 +267,8 @@ def main():
                        "p99": round(float(np.percentile(f16_ms, 99)), 3)},
         "fp32_store": {"gups": round(gups32, 3), "ms_per_step": round(statistics.mean(f32_ms), 3),
                        "backproject_ms": round(bp_ms32, 3),
-                       "roofline_frac": round(ach32 / HBM_PEAK_GBPS, 4)},
+                       "roofline_frac": round(ach32 / HBM_PEAK_GBPS, 4),
+                       "traffic": read_traffic("fp32_bytes_per_launch")},
         "fp16_vs_fp32_gups": round(gups / gups32, 3),
         "rel_rmse_fp16_vs_fp32": rel_rmse_16,
         "bf16_store": {"gups": round(g.updates / (statistics.mean(b16_ms) * 1e-3) / 1e9, 3),
@@ -265,7 +277,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(ach16, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(ach16 / HBM_PEAK_GBPS, 4),
                      "traffic": traffic,
-                     "kernel": "backproject_lds_kernel<fp16, TY=32, HMAX=112>",
+                     "kernel": "backproject_lds_kernel<fp16 packed, TY=32, HMAX=96>",
                      "avg_ms": round(bp_ms, 3)},
         "cpu_baseline": cpu,
     }

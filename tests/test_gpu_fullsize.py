@@ -64,11 +64,21 @@ def test_full_size_fp16_storage_crops(ctr, orc, dev):
     vol = ctr.backproject(staged, g, ctr.F16, view_scale=vs, scale=1e-3)
     torch.cuda.synchronize()
     qh = q.cpu().numpy()
+    peak = None
     for box in CROPS_512[:1] + CROPS_512[2:]:
         ref = orc.backproject_cone_region(qh, g.source_distance, g.detector_distance,
                                           g.detector_width, g.volume_width, 1.0, 0.0, 1e-3,
                                           (g.X, g.Y, g.Z), box)
         assert rel_rmse(_crop(vol, box), ref) < 1e-3
+        peak = max(peak or 0.0, float(np.abs(ref).max()))
+    # corner crop: its projections leave the detector sideways in some views
+    # (the values there are small, so the bound is relative to the centre's
+    # peak rather than to the crop itself)
+    box = CROPS_512[1]
+    ref = orc.backproject_cone_region(qh, g.source_distance, g.detector_distance,
+                                      g.detector_width, g.volume_width, 1.0, 0.0, 1e-3,
+                                      (g.X, g.Y, g.Z), box)
+    assert np.abs(_crop(vol, box) - ref).max() < 1e-3 * peak
 
 
 def test_full_size_constant_projection_closed_form(ctr, dev):
@@ -113,23 +123,34 @@ def test_full_size_filter_rows_sample(ctr, orc, dev):
 
 
 def test_backproject_kernel_variants_agree(ctr, dev, monkeypatch):
-    """The LDS-staged kernel and the direct-tap kernel agree to FP32
-    rounding: both take their coordinates from the same FP64 column set-up
-    and the LDS footprint holds exactly the taps the direct kernel reads; only
-    the y-run length (32 vs 16 voxels) changes where the FP32 row step
-    starts; the gap (about 2e-6 of the peak) is FP32 rounding of that step,
-    fifty times inside the 1e-4 parity gate."""
+    """The LDS-staged kernel and the direct-tap kernel agree over the whole
+    256^3 volume, including the corner tiles whose projections leave the
+    detector sideways in some views (zero texels there).  fp32 and bf16 agree
+    to FP32 rounding: both kernels take their coordinates from the same FP64
+    column set-up and the LDS footprint holds exactly the taps the direct
+    kernel reads; only the y-run length (32 vs 16 voxels) changes where the
+    FP32 row step starts (about 2e-6 of the peak).  fp16 is interpolated in
+    packed f16 arithmetic by the LDS kernel and in FP32 by the direct one:
+    the gap is f16 rounding of the lerped pair (about 4e-4 rel RMSE on
+    white-noise projections, 5e-5 on a phantom scan)."""
     import torch
     g = ctr.config_geometry("fdk_256_s")
     q = _random_views(ctr, g, dev, 5)
-    for dt in (ctr.F32, ctr.F16):
+    for dt in (ctr.F32, ctr.BF16, ctr.F16):
         staged, vs = ctr.stage_projections(q, g, dt)
         monkeypatch.setenv("CTR_BACKPROJECT_KERNEL", "lds")
         a = ctr.backproject(staged, g, dt, view_scale=vs, scale=0.01)
         monkeypatch.setenv("CTR_BACKPROJECT_KERNEL", "gather")
         b = ctr.backproject(staged, g, dt, view_scale=vs, scale=0.01)
         torch.cuda.synchronize()
-        assert rel_max(a.cpu().numpy(), b.cpu().numpy()) < 5e-6
+        a, b = a.cpu().numpy(), b.cpu().numpy()
+        print(f"[variants] dtype {dt}: max {rel_max(a, b):.3e} rmse {rel_rmse(a, b):.3e}",
+              flush=True)
+        if dt == ctr.F16:
+            assert rel_max(a, b) < 2e-3, dt
+            assert rel_rmse(a, b) < 8e-4, dt
+        else:
+            assert rel_max(a, b) < 5e-6, dt
 
 
 def test_lds_kernel_falls_back_on_wide_footprints(ctr, orc, dev, monkeypatch):
