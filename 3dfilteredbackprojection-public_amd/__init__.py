@@ -82,6 +82,10 @@ SIGNATURES = {
     "ctr_rebin_fan_to_parallel": [_v, _v, _i, _i, _d, _d, _d, _d, _v, _i, _d],
     "ctr_render_ellipsoids": [_v, _v, _v, _i, _v, _i, _i, _i, _d],
     "ctr_poisson_log": [_v, _v, _sz, _d, _u],
+    "ctr_decode_frames": [_v, _v, _i, _i, _i, _i, _i, _v, _i, _i, _i, _i, _i],
+    "ctr_min_max": [_v, _v, _v, _sz, _v],
+    "ctr_normalize_range": [_v, _v, _sz, _v],
+    "ctr_slices_unorm16": [_v, _v, _i, _i, _i, _i, _i, _v],
 }
 _RESTYPE = {"ctr_last_error": ctypes.c_char_p}
 
@@ -393,6 +397,52 @@ def clip_negative(data, stream=None):
     _f32c(data)
     check(lib().ctr_clip_negative(stream_handle(stream), ptr(data), data.numel()))
     return data
+
+
+def decode_frames(raw, file_w, file_h, channels, bytes_per_channel, out, target_w, target_h,
+                  first_view=0, layout=PROJECTIONS, stream=None):
+    """Raw frames (a contiguous uint8 device tensor of n_frames frames) ->
+    views first_view.. of `out`, a float32 (N, target_h, target_w) projection
+    stack or (target_h, N, target_w) sinogram stack (ctr_decode_frames)."""
+    _f32c(out)
+    frame_bytes = file_w * file_h * channels * bytes_per_channel
+    assert raw.is_contiguous() and raw.numel() % frame_bytes == 0
+    n_frames = raw.numel() // frame_bytes
+    n_views = out.shape[0] if layout == PROJECTIONS else out.shape[1]
+    check(lib().ctr_decode_frames(stream_handle(stream), ptr(raw), file_w, file_h, channels,
+                                  bytes_per_channel, n_frames, ptr(out), layout, target_w,
+                                  target_h, n_views, first_view))
+    return out
+
+
+def min_max(data, ws=None, stream=None):
+    """(min, max) of a float32 device tensor as a 2-element device tensor."""
+    torch = _torch()
+    _f32c(data)
+    out = torch.empty(2, dtype=torch.float32, device=data.device)
+    check(lib().ctr_min_max(stream_handle(stream), ws.handle if ws else None, ptr(data),
+                            data.numel(), ptr(out)))
+    return out
+
+
+def normalize_range(data, min_max_tensor, stream=None):
+    _f32c(data)
+    check(lib().ctr_normalize_range(stream_handle(stream), ptr(data), data.numel(),
+                                    ptr(min_max_tensor)))
+    return data
+
+
+def slices_unorm16(volume, y0=0, count=None, stream=None):
+    """y-slices of a (Z, Y, X) volume as unorm16 values, returned as an
+    int16 device tensor (count, Z, X) holding the uint16 bit patterns."""
+    torch = _torch()
+    _f32c(volume)
+    Z, Y, X = volume.shape
+    count = Y - y0 if count is None else count
+    out = torch.empty((count, Z, X), dtype=torch.int16, device=volume.device)
+    check(lib().ctr_slices_unorm16(stream_handle(stream), ptr(volume), X, Y, Z, y0, count,
+                                   ptr(out)))
+    return out
 
 
 def rebin_fan_to_parallel(fan, geom, nu_par, parallel_width, stream=None):

@@ -13,12 +13,15 @@
 // reference's 2^ceil(log2 nu)) lets callers pick the zero-padded FFT length;
 // `staged_dtype` selects FP32 or FP16/BF16 filtered-projection storage for
 // the backprojector; the cone forward projector takes an optional explicit
-// detector size.  File I/O, slice viewers and histogram normalisation
-// (FBP3D.cpp:74-195,780-826) are outside this library's scope and not
-// declared; ComputeProgram members have no counterpart.
+// detector size.  Projection ingest, slice output and the volume range
+// helpers (FBP3D.cpp:74-195,780-826) decode / convert on the device (see
+// fbp3d_io.cpp); blur_volume, the read_volume / write_projections /
+// read_sinograms / write_sinograms declarations the reference never defines
+// and the ComputeProgram members have no counterpart.
 #pragma once
 
 #include <memory>
+#include <string>
 
 #include "FBP2D.h"
 #include "FastFourierTransformGPU.h"
@@ -39,6 +42,20 @@ public:
     void render_ellipsoid(glm::vec3 center, glm::vec3 axis_lengths,
                           glm::vec3 rotation_angles_radian, float value);
     void generate_shepplogan(int x_size, int y_size, int z_size);
+
+    // FBP3D.cpp:74-176: frames named "...NNNN.ext" (NNNN = projection index),
+    // channel 0 of file_channel_count interleaved channels of
+    // byte_per_channel (1 or 2) bytes, resampled to the target size.
+    void read_projections(const std::string &upper_diretory, int file_width, int file_height,
+                          int file_channel_count, int byte_per_channel,
+                          int target_projection_width, int target_projection_height,
+                          int projection_count);
+    void read_projections_as_sinograms(const std::string &upper_diretory, int file_width,
+                                       int file_height, int file_channel_count,
+                                       int byte_per_channel, int target_projection_width,
+                                       int target_projection_height, int projection_count);
+    // FBP3D.cpp:178-195: one raw unorm16 file per y-slice.
+    void write_volume(std::string upper_directory);
 
     void free_volume();
     void free_projections();
@@ -114,6 +131,13 @@ public:
     void load_projection_from_sinograms(int projection_index, Texture2D &target_texture_out);
     void store_projection_to_sinograms(Texture2D &soruce_texture, int projection_index);
 
+    // FBP3D.cpp:780-826: 1-texel textures carry the value's float bits.
+    void compute_max_value_of_volume(Texture1D &target_int_texture);
+    void compute_min_value_of_volume(Texture1D &target_int_texture);
+    void normalize_histogram();
+    void normalize_histogram(Texture1D &source_min_int_texture,
+                             Texture1D &source_max_int_texture);
+
     std::shared_ptr<FBP2D> fbp_solver = std::make_shared<FBP2D>();
     std::shared_ptr<FFFT> fft_solver = std::make_shared<FFFT>();
 
@@ -132,6 +156,8 @@ private:
     int filtered_fft_length_ = 0;  // FFT length of the last apply_filter_* call
     DeviceBuffer staged_;
     DeviceBuffer view_scale_;
+    DeviceBuffer minmax_;
+    static constexpr size_t kSliceIndexDigits = 6;  // FBP3D.h:161
 };
 
 }  // namespace ctrecon

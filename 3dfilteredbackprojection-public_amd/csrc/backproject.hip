@@ -426,53 +426,85 @@ __global__ __launch_bounds__(256) void backproject_kernel(BpJob J)
 //     range), and the run is rescaled by scale * max(view_scale) at the
 //     store.
 // ---------------------------------------------------------------------------
-constexpr int kLdsPitch = 65;   // dwords per LDS row: 64 staged + 1 against bank aliasing
+// LDS geometry per staged format: row pitch in dwords and rows per buffer.
+// The pitch is 64 staged texels plus padding that staggers the banks of
+// neighbouring rows.  The lanes of a wave read taps from an 8 x 8 patch of
+// columns, i.e. from texels within about +-10 columns and +-2 rows of each
+// other; with 32 banks a row shift of s banks makes (row + dr, col + du)
+// collide when s dr + du = 0 mod 32: s = 1 (pitch 65) collides for every
+// (1, -1) neighbour, s = 5 (69) only for |du| >= 5 pairs two rows apart,
+// s = 11 (75) only for |du| >= 10.  Two buffers of `rows` rows bound the
+// workgroups per CU (160 KB of LDS): 84 rows at pitch 75 give three, 70 rows
+// at pitch 69 four.  Measured on 512^3 x 720 (GUPS, pitch/rows):
+//   fp32  65/96 1712, 75/84 2662, 75/64 2598, 69/70 2386
+//   fp16  65/96 2664, 75/84 2921, 72/66 3065, 69/70 3243
+// fp32 reads four taps per voxel and is the most sensitive to conflicts;
+// 16-bit pairs read two and profit more from the fourth workgroup.  Every
+// 32-voxel footprint of the 512^3 scan fits 84 rows, 99.9% fit 70 (98% of
+// the 1440^3 scan); larger ones take the direct-tap pass.
+#ifndef CTR_LDS_PITCH32
+#define CTR_LDS_PITCH32 75
+#endif
+#ifndef CTR_LDS_ROWS32
+#define CTR_LDS_ROWS32 84
+#endif
+#ifndef CTR_LDS_PITCH16
+#define CTR_LDS_PITCH16 69
+#endif
+#ifndef CTR_LDS_ROWS16
+#define CTR_LDS_ROWS16 70
+#endif
+#ifndef CTR_LDS_TY
+#define CTR_LDS_TY 32
+#endif
 constexpr int kLdsCols = 64;    // texels staged per row (one per lane)
 constexpr int kRectChunk = 128; // views per refill of the rectangle table
 constexpr int kF16Packed = 16;  // tile variant: fp16 pairs with packed f16 arithmetic
 
+template <int DT>
+struct LdsConfig {
+    static constexpr int kPitch = CTR_LDS_PITCH16, kRows = CTR_LDS_ROWS16;
+};
+template <>
+struct LdsConfig<CTR_F32> {
+    static constexpr int kPitch = CTR_LDS_PITCH32, kRows = CTR_LDS_ROWS32;
+};
+
 enum RectMode { RECT_SKIP = 0, RECT_LDS = 1, RECT_DIRECT = 2 };
 
 // LDS of the kernel: the rectangle table (kRectChunk + 1 int4), then two row
-// buffers of HMAX rows.
-template <int HMAX>
+// buffers.
+template <int DT>
 struct LdsLayout {
+    static constexpr int kPitch = LdsConfig<DT>::kPitch, kRows = LdsConfig<DT>::kRows;
     static constexpr int kRectBytes = (kRectChunk + 1) * 16;
-    static constexpr int kBufDwords = HMAX * kLdsPitch;
+    static constexpr int kBufDwords = kRows * kPitch;
     static constexpr int kDwords = kRectBytes / 4 + 2 * kBufDwords;
     static_assert((2 * kBufDwords) % 4 == 0, "buffers are cleared as uint4");
+    static_assert(kPitch >= kLdsCols + 1 && kPitch + 1 <= 255, "ds_read2 offsets");
 };
 
 // Taps at LDS byte address `addr`: rows I (top) and I + 1 (bot), columns
 // (u, u + 1) as fp32.
 template <int DT>
-__device__ inline void lds_taps(const unsigned *L, unsigned addr, float2 &top, float2 &bot);
-
-template <>
-__device__ inline void lds_taps<CTR_F32>(const unsigned *L, unsigned addr, float2 &top,
-                                         float2 &bot)
+__device__ inline void lds_taps(const unsigned *L, unsigned addr, float2 &top, float2 &bot)
 {
-    const float *F = reinterpret_cast<const float *>(reinterpret_cast<const char *>(L) + addr);
-    top = make_float2(F[0], F[1]);
-    bot = make_float2(F[kLdsPitch], F[kLdsPitch + 1]);
-}
-
-template <>
-__device__ inline void lds_taps<CTR_F16>(const unsigned *L, unsigned addr, float2 &top,
-                                         float2 &bot)
-{
-    const unsigned *P = reinterpret_cast<const unsigned *>(reinterpret_cast<const char *>(L) + addr);
-    top = f16_pair(P[0]);
-    bot = f16_pair(P[kLdsPitch]);
-}
-
-template <>
-__device__ inline void lds_taps<CTR_BF16>(const unsigned *L, unsigned addr, float2 &top,
-                                          float2 &bot)
-{
-    const unsigned *P = reinterpret_cast<const unsigned *>(reinterpret_cast<const char *>(L) + addr);
-    top = bf16_pair(P[0]);
-    bot = bf16_pair(P[kLdsPitch]);
+    constexpr int P = LdsConfig<DT>::kPitch;
+    if constexpr (DT == CTR_F32) {
+        const float *F = reinterpret_cast<const float *>(reinterpret_cast<const char *>(L) + addr);
+        top = make_float2(F[0], F[1]);
+        bot = make_float2(F[P], F[P + 1]);
+    } else {
+        const unsigned *Q =
+            reinterpret_cast<const unsigned *>(reinterpret_cast<const char *>(L) + addr);
+        if constexpr (DT == CTR_BF16) {
+            top = bf16_pair(Q[0]);
+            bot = bf16_pair(Q[P]);
+        } else {
+            top = f16_pair(Q[0]);
+            bot = f16_pair(Q[P]);
+        }
+    }
 }
 
 // Detector rectangle of one tile for view n: (u0, v0, rows, mode), staged
@@ -517,6 +549,7 @@ __device__ inline int4 read_rect(const int4 *tab, int c)
 // buffer resource spanning exactly the view (raw 32-bit format), with the
 // row offset in an SGPR and the lane's column in the VGPR offset: no VALU
 // work per row, and any address outside the view would read zero.
+template <int PITCH>
 __device__ inline void stage_rect(const char *view, unsigned view_bytes, int pitch_u,
                                   int pitch_v, int u0, int v0, int h, unsigned *buf, int wave,
                                   int lane)
@@ -530,7 +563,7 @@ __device__ inline void stage_rect(const char *view, unsigned view_bytes, int pit
     for (int r = wave; r < h; r += 4) {
         const int srow = min(max(v0 + r, 0), pitch_v - 1);
         __builtin_amdgcn_raw_ptr_buffer_load_lds(
-            rsrc, (__attribute__((address_space(3))) void *)(buf + r * kLdsPitch), 4, col4,
+            rsrc, (__attribute__((address_space(3))) void *)(buf + r * PITCH), 4, col4,
             (unsigned)srow * row_bytes, 0, 0);
     }
 }
@@ -573,7 +606,7 @@ __device__ inline void lds_run_accumulate(float (&acc)[TY], const LdsRun &R, con
             const unsigned *P =
                 reinterpret_cast<const unsigned *>(reinterpret_cast<const char *>(lds) + addr);
             const half2_t a = __builtin_bit_cast(half2_t, P[0]);
-            const half2_t b = __builtin_bit_cast(half2_t, P[kLdsPitch]);
+            const half2_t b = __builtin_bit_cast(half2_t, P[LdsConfig<DT>::kPitch]);
             const _Float16 w16 = (_Float16)wv;
             const half2_t w = {w16, w16};
             const half2_t lr = __builtin_elementwise_fma(w, b - a, a);
@@ -625,11 +658,12 @@ __device__ inline void lds_run_accumulate(float (&acc)[TY], const LdsRun &R, con
     }
 }
 
-template <int DT, int TY, int HMAX>
+template <int DT, int TY>
 __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int bz,
                                             unsigned *lds)
 {
-    using L = LdsLayout<HMAX>;
+    using L = LdsLayout<DT>;
+    constexpr int HMAX = L::kRows;
     constexpr int kTapDT = DT == kF16Packed ? CTR_F16 : DT;  // staged texel format
     int4 *rects = reinterpret_cast<int4 *>(lds);
     unsigned *bufs = lds + L::kRectBytes / 4;
@@ -679,7 +713,7 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
 
     int4 cur = read_rect(rects, 0);
     if (cur.w == RECT_LDS)
-        stage_rect(vbase, view_bytes32, pitch_u, pitch_v, cur.x, cur.y, cur.z, bufs, wave, lane);
+        stage_rect<L::kPitch>(vbase, view_bytes32, pitch_u, pitch_v, cur.x, cur.y, cur.z, bufs, wave, lane);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
 
@@ -692,7 +726,7 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
         if (n + 1 < N) {
             nxt = read_rect(rects, c + 1);
             if (nxt.w == RECT_LDS)
-                stage_rect(vbase + (size_t)(n + 1) * view_bytes, view_bytes32, pitch_u, pitch_v,
+                stage_rect<L::kPitch>(vbase + (size_t)(n + 1) * view_bytes, view_bytes32, pitch_u, pitch_v,
                            nxt.x, nxt.y, nxt.z, bufs + ((n + 1) & 1) * L::kBufDwords, wave,
                            lane);
         }
@@ -751,23 +785,23 @@ __device__ inline int xcd_tile_order(int b, int nblocks)
     return xcd * q + min(xcd, r) + k;
 }
 
-template <int DT, int TY, int HMAX>
+template <int DT, int TY>
 __global__ __launch_bounds__(256) void backproject_lds_kernel(BpJob J)
 {
-    __shared__ __align__(16) unsigned lds[LdsLayout<HMAX>::kDwords];
+    __shared__ __align__(16) unsigned lds[LdsLayout<DT>::kDwords];
     const int t = xcd_tile_order(blockIdx.x, gridDim.x);
     const int bx = t % J.gx;
     const int by = (t / J.gx) % J.gy;
     const int bz = t / (J.gx * J.gy);
-    backproject_lds_tile<DT, TY, HMAX>(J, bx, by, bz, lds);
+    backproject_lds_tile<DT, TY>(J, bx, by, bz, lds);
 }
 
 // Packed launch: the flat tile index selects the job (the last job whose
 // block_begin it reaches) and the tile inside it.
-template <int DT, int TY, int HMAX>
+template <int DT, int TY>
 __global__ __launch_bounds__(256) void backproject_lds_packed_kernel(BpBatch B)
 {
-    __shared__ __align__(16) unsigned lds[LdsLayout<HMAX>::kDwords];
+    __shared__ __align__(16) unsigned lds[LdsLayout<DT>::kDwords];
     const int b = xcd_tile_order(blockIdx.x, gridDim.x);
     int j = 0;
 #pragma unroll 1
@@ -778,7 +812,7 @@ __global__ __launch_bounds__(256) void backproject_lds_packed_kernel(BpBatch B)
     const int bx = local % J.gx;
     const int by = (local / J.gx) % J.gy;
     const int bz = local / (J.gx * J.gy);
-    backproject_lds_tile<DT, TY, HMAX>(J, bx, by, bz, lds);
+    backproject_lds_tile<DT, TY>(J, bx, by, bz, lds);
 }
 
 // Kernel choice: the LDS-staged kernel for volumes at least 32 voxels along
@@ -793,7 +827,7 @@ static int variant_override()
     return 0;
 }
 
-constexpr int kTY = 32, kHMAX = 96;  // LDS kernel: y-run and rows per buffer
+constexpr int kTY = CTR_LDS_TY;  // LDS kernel: voxels per thread along y
 
 static BpJob make_job(const void *staged, const StagedLayout &L, int nu, int nv, GeomPtrs geom,
                       const float *view_scale, int n_views, float *vol, int X, int Y, int Z,
@@ -821,7 +855,7 @@ static BpJob make_job(const void *staged, const StagedLayout &L, int nu, int nv,
     J.gy = (Z + 15) / 16;
     J.foot = nullptr;
     J.block_begin = 0;
-    J.lds_pitch4 = kLdsPitch * 4u;
+    J.lds_pitch4 = 0;  // set per staged format at launch
     return J;
 }
 
@@ -861,15 +895,29 @@ static bool f16_weights_fit(const BpJob &J, int geometry, double sod, double vol
 }
 
 template <int DT>
+static void launch_lds(hipStream_t s, BpJob J, unsigned blocks)
+{
+    J.lds_pitch4 = LdsConfig<DT>::kPitch * 4u;
+    backproject_lds_kernel<DT, kTY><<<blocks, 256, 0, s>>>(J);
+}
+
+template <int DT>
+static void launch_lds_packed(hipStream_t s, BpBatch B, unsigned blocks)
+{
+    for (int q = 0; q < B.count; ++q) B.job[q].lds_pitch4 = LdsConfig<DT>::kPitch * 4u;
+    backproject_lds_packed_kernel<DT, kTY><<<blocks, 256, 0, s>>>(B);
+}
+
+template <int DT>
 static int launch_bp(hipStream_t s, const BpJob &J, bool packed_f16)
 {
     if (use_lds_kernel(J.Y)) {
         const long long blocks = (long long)J.gx * J.gy * ((J.Y + kTY - 1) / kTY);
         CTR_REQUIRE(blocks < (1LL << 31), CTR_ERR_INVALID_ARGUMENT, "volume too large");
         if (DT == CTR_F16 && packed_f16)
-            backproject_lds_kernel<kF16Packed, kTY, kHMAX><<<(unsigned)blocks, 256, 0, s>>>(J);
+            launch_lds<kF16Packed>(s, J, (unsigned)blocks);
         else
-            backproject_lds_kernel<DT, kTY, kHMAX><<<(unsigned)blocks, 256, 0, s>>>(J);
+            launch_lds<DT>(s, J, (unsigned)blocks);
         CTR_CHECK_LAUNCH();
         return CTR_OK;
     }
@@ -1006,19 +1054,14 @@ int ctr_backproject_packed(void *stream, ctr_workspace *ws, int staged_dtype,
     }
     CTR_REQUIRE(blocks < (1LL << 31), CTR_ERR_INVALID_ARGUMENT, "packed launch too large");
     switch (staged_dtype) {
-    case CTR_F32:
-        backproject_lds_packed_kernel<CTR_F32, kTY, kHMAX><<<(unsigned)blocks, 256, 0, s>>>(B);
-        break;
+    case CTR_F32: launch_lds_packed<CTR_F32>(s, B, (unsigned)blocks); break;
     case CTR_F16:
         if (packed)
-            backproject_lds_packed_kernel<kF16Packed, kTY, kHMAX>
-                <<<(unsigned)blocks, 256, 0, s>>>(B);
+            launch_lds_packed<kF16Packed>(s, B, (unsigned)blocks);
         else
-            backproject_lds_packed_kernel<CTR_F16, kTY, kHMAX><<<(unsigned)blocks, 256, 0, s>>>(B);
+            launch_lds_packed<CTR_F16>(s, B, (unsigned)blocks);
         break;
-    default:
-        backproject_lds_packed_kernel<CTR_BF16, kTY, kHMAX><<<(unsigned)blocks, 256, 0, s>>>(B);
-        break;
+    default: launch_lds_packed<CTR_BF16>(s, B, (unsigned)blocks); break;
     }
     CTR_CHECK_LAUNCH();
     return CTR_OK;

@@ -34,11 +34,12 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
 // Workspace slots.  Each slot is one device buffer grown on demand.
 enum Slot {
     SLOT_GEOMETRY = 0,   // per-view projective rows, 3 float4 per view
-    SLOT_VIEW_MAX = 1,   // per-view max |q| as uint bits, 1 uint per view
+    SLOT_VIEW_MAX = 1,   // per-row max |q| of the filter output (16-bit staging)
     SLOT_STAGE_F32 = 2,  // fp32 staged projections before rounding to 16 bit
     SLOT_ELLIPSOIDS = 3, // ellipsoid records for the phantom renderer
     SLOT_FOOTPRINT = 4,  // per (tile column, view) detector footprint bounds
-    SLOT_COUNT = 5
+    SLOT_REDUCE = 5,     // min / max reduction keys
+    SLOT_COUNT = 6
 };
 
 // Resolves NULL to the per-device default workspace.

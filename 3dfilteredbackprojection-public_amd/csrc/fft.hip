@@ -190,7 +190,57 @@ struct FilterJob {
     float *out;
     float *staged;
     int pitch_u, pitch_v;
+    // optional per-row max |value| of the output (indexed like the input
+    // rows), reduced per view afterwards
+    float *row_max;
 };
+
+// Stores the row pair's max |value|: wave reduction, then one LDS pass over
+// the (at most 16) waves of the workgroup; no atomics.
+__device__ inline void note_row_max(const FilterJob &J, float ma, float mb, int r0, bool has1)
+{
+    __shared__ float part[2][16];
+    for (int off = 32; off > 0; off >>= 1) {
+        ma = fmaxf(ma, __shfl_xor(ma, off));
+        mb = fmaxf(mb, __shfl_xor(mb, off));
+    }
+    const int wave = threadIdx.x >> 6, waves = (blockDim.x + 63) >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        part[0][wave] = ma;
+        part[1][wave] = mb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < waves; ++w) {
+            ma = fmaxf(ma, part[0][w]);
+            mb = fmaxf(mb, part[1][w]);
+        }
+        J.row_max[r0] = ma;
+        if (has1) J.row_max[r0 + 1] = mb;
+    }
+}
+
+// view_scale[n] = max over the rows of view n of row_max (1 when 0).
+__global__ __launch_bounds__(256) void view_max_from_rows_kernel(const float *row_max, int nv,
+                                                                 int n_views, int layout,
+                                                                 float *view_scale)
+{
+    const int n = blockIdx.x;
+    float m = 0.0f;
+    for (int iv = threadIdx.x; iv < nv; iv += blockDim.x) {
+        const size_t r = layout == CTR_LAYOUT_PROJECTIONS ? (size_t)n * nv + iv
+                                                          : (size_t)iv * n_views + n;
+        m = fmaxf(m, row_max[r]);
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    __shared__ float part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
+        view_scale[n] = m > 0.0f ? m : 1.0f;
+    }
+}
 
 __device__ inline void row_coords(const FilterJob &J, int r, int &view, int &iv)
 {
@@ -277,11 +327,16 @@ __global__ __launch_bounds__(kFftThreads) void filter_pairs_kernel(FilterJob J)
         size_t vstride = (size_t)J.pitch_u * J.pitch_v;
         float *oa = J.staged + (size_t)view0 * vstride + (size_t)(iv0 + 1) * J.pitch_u + 1;
         float *ob = J.staged + (size_t)view1 * vstride + (size_t)(iv1 + 1) * J.pitch_u + 1;
+        float ma = 0.0f, mb = 0.0f;
         for (int e = threadIdx.x; e < J.width; e += blockDim.x) {
             float2 v = L[e];
-            oa[e] = v.x * inv_n;
-            if (has1) ob[e] = v.y * inv_n;
+            const float a = v.x * inv_n, b = v.y * inv_n;
+            oa[e] = a;
+            if (has1) ob[e] = b;
+            ma = fmaxf(ma, fabsf(a));
+            mb = fmaxf(mb, fabsf(b));
         }
+        if (J.row_max) note_row_max(J, ma, mb, r0, has1);
     }
 }
 
@@ -510,16 +565,21 @@ __global__ __launch_bounds__((1 << LOG2N) / 8) void filter_stockham_kernel(Filte
         oa = J.staged + (size_t)view0 * vstride + (size_t)(iv0 + 1) * J.pitch_u + 1;
         ob = J.staged + (size_t)view1 * vstride + (size_t)(iv1 + 1) * J.pitch_u + 1;
     }
+    float ma = 0.0f, mb = 0.0f;
 #pragma unroll
     for (int b = 0; b < 8 / RF; ++b)
 #pragma unroll
         for (int r = 0; r < RF; ++r) {
             const int e = tid + b * T + r * (N / RF);
             if (e < J.width) {
-                oa[e] = v[b * RF + r].x * inv_n;
-                if (has1) ob[e] = v[b * RF + r].y * inv_n;
+                const float x = v[b * RF + r].x * inv_n, y = v[b * RF + r].y * inv_n;
+                oa[e] = x;
+                if (has1) ob[e] = y;
+                ma = fmaxf(ma, fabsf(x));
+                mb = fmaxf(mb, fabsf(y));
             }
         }
+    if (J.row_max) note_row_max(J, ma, mb, r0, has1);
 }
 
 static int launch_filter_pairs(hipStream_t s, const FilterJob &J, int log2n)
@@ -552,18 +612,22 @@ static int launch_filter_pairs(hipStream_t s, const FilterJob &J, int log2n)
 // ---------------------------------------------------------------------------
 // Staged layout helpers
 // ---------------------------------------------------------------------------
-// Zeroes the border texels of an fp32 staged buffer (rows 0, nv+1, nv+2 and
-// columns 0, nu+1 .. pitch_u-1 of every view).
-__global__ void zero_border_kernel(float *st, int nu, int nv, int n_views,
-                                   int pitch_u, int pitch_v)
+// Zeroes the border texels of an fp32 staged buffer: rows 0, nv+1, nv+2 of
+// every view entirely, columns 0 and nu+1 .. pitch_u-1 of the others.  One
+// 64-thread workgroup per staged row (pitch_u - nu <= 6 border columns).
+__global__ __launch_bounds__(64) void zero_border_kernel(float *st, int nu, int nv, int pitch_u,
+                                                         int pitch_v)
 {
-    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    long long total = (long long)pitch_u * pitch_v * n_views;
-    if (idx >= total) return;
-    int c = (int)(idx % pitch_u);
-    int r = (int)((idx / pitch_u) % pitch_v);
-    bool interior = r >= 1 && r <= nv && c >= 1 && c <= nu;
-    if (!interior) st[idx] = 0.0f;
+    const unsigned row = blockIdx.x;  // staged row over all views
+    const int r = (int)(row % (unsigned)pitch_v);
+    float *p = st + (size_t)row * pitch_u;
+    if (r == 0 || r > nv) {
+        for (int c = threadIdx.x; c < pitch_u; c += 64) p[c] = 0.0f;
+    } else if (threadIdx.x == 0) {
+        p[0] = 0.0f;
+    } else if (nu + (int)threadIdx.x < pitch_u) {
+        p[nu + threadIdx.x] = 0.0f;
+    }
 }
 
 // Copies plain projections (either layout) into an fp32 staged buffer,
@@ -621,28 +685,38 @@ __device__ inline unsigned short f32_to_f16_bits(float f)
 
 // fp32 staged -> 16-bit pair texels: texel c of a row holds
 // (q[c], q[c+1]) / view_scale, so one aligned 32-bit load gives both
-// horizontal bilinear taps.
-__global__ void quantize_pairs_kernel(const float *st, int pitch_u, int pitch_v,
-                                      int n_views, const float *view_scale,
-                                      unsigned *out, int bf16)
+// horizontal bilinear taps.  One workgroup per staged row, four texels per
+// thread (pitch_u is a multiple of 4).  A view whose max is 0 gets scale 1.
+__device__ inline unsigned pack_pair(float a, float b, int bf16)
 {
-    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    long long total = (long long)pitch_u * pitch_v * n_views;
-    if (idx >= total) return;
-    int c = (int)(idx % pitch_u);
-    int n = (int)(idx / ((long long)pitch_u * pitch_v));
-    float inv = 1.0f / view_scale[n];
-    float a = st[idx] * inv;
-    float b = c + 1 < pitch_u ? st[idx + 1] * inv : 0.0f;
-    unsigned lo, hi;
-    if (bf16) {
-        lo = f32_to_bf16_rne(a);
-        hi = f32_to_bf16_rne(b);
-    } else {
-        lo = f32_to_f16_bits(a);
-        hi = f32_to_f16_bits(b);
+    return bf16 ? (unsigned)f32_to_bf16_rne(a) | ((unsigned)f32_to_bf16_rne(b) << 16)
+                : (unsigned)f32_to_f16_bits(a) | ((unsigned)f32_to_f16_bits(b) << 16);
+}
+
+__global__ __launch_bounds__(256) void quantize_pairs_kernel(const float *st, int pitch_u,
+                                                             int pitch_v, float *view_scale,
+                                                             unsigned *out, int bf16)
+{
+    const unsigned row = blockIdx.x;  // staged row over all views
+    const unsigned n = row / (unsigned)pitch_v;
+    float vs = view_scale[n];
+    if (!(vs > 0.0f)) {
+        vs = 1.0f;
+        if (threadIdx.x == 0 && row % (unsigned)pitch_v == 0) view_scale[n] = 1.0f;
     }
-    out[idx] = lo | (hi << 16);
+    const float inv = 1.0f / vs;
+    const float *src = st + (size_t)row * pitch_u;
+    unsigned *dst = out + (size_t)row * pitch_u;
+    for (int c = 4 * threadIdx.x; c < pitch_u; c += 4 * blockDim.x) {
+        const float4 a = *reinterpret_cast<const float4 *>(src + c);
+        const float e = c + 4 < pitch_u ? src[c + 4] : 0.0f;
+        uint4 o;
+        o.x = pack_pair(a.x * inv, a.y * inv, bf16);
+        o.y = pack_pair(a.y * inv, a.z * inv, bf16);
+        o.z = pack_pair(a.z * inv, a.w * inv, bf16);
+        o.w = pack_pair(a.w * inv, e * inv, bf16);
+        *reinterpret_cast<uint4 *>(dst + c) = o;
+    }
 }
 
 static inline unsigned grid1d(long long total, int block)
@@ -650,20 +724,23 @@ static inline unsigned grid1d(long long total, int block)
     return (unsigned)((total + block - 1) / block);
 }
 
-// fp32 staged (borders zero) -> caller's staged buffer in `dtype`.
+// fp32 staged (borders zero) -> caller's staged buffer in `dtype`.  With
+// `have_max` the per-view maxima are already in view_scale (from the filter).
 static int finish_staged(hipStream_t s, const float *st32, int nu, int nv, int n_views,
-                         void *staged, int dtype, float *view_scale)
+                         void *staged, int dtype, float *view_scale, bool have_max)
 {
     StagedLayout L = staged_layout(nu, nv, n_views, dtype);
     if (dtype == CTR_F32) return CTR_OK;
     CTR_REQUIRE(view_scale, CTR_ERR_NULL_POINTER,
                 "view_scale is required for 16-bit staged projections");
-    view_max_kernel<<<n_views, 256, 0, s>>>(st32, L.view, view_scale);
-    CTR_CHECK_LAUNCH();
-    long long total = (long long)L.view * n_views;
-    quantize_pairs_kernel<<<grid1d(total, 256), 256, 0, s>>>(
-        st32, L.pitch_u, L.pitch_v, n_views, view_scale, (unsigned *)staged,
-        dtype == CTR_BF16);
+    if (!have_max) {
+        view_max_kernel<<<n_views, 256, 0, s>>>(st32, L.view, view_scale);
+        CTR_CHECK_LAUNCH();
+    }
+    const long long rows = (long long)L.pitch_v * n_views;
+    CTR_REQUIRE(rows < (1LL << 31), CTR_ERR_INVALID_ARGUMENT, "too many staged rows");
+    quantize_pairs_kernel<<<(unsigned)rows, 256, 0, s>>>(st32, L.pitch_u, L.pitch_v, view_scale,
+                                                         (unsigned *)staged, dtype == CTR_BF16);
     CTR_CHECK_LAUNCH();
     return CTR_OK;
 }
@@ -988,10 +1065,22 @@ int ctr_filter_projections(void *stream, ctr_workspace *ws, const float *proj, i
         if ((rc = workspace_get(w, SLOT_STAGE_F32, L.bytes, &p))) return rc;
         st32 = (float *)p;
     }
-    long long total = (long long)L.view * n_views;
-    zero_border_kernel<<<grid1d(total, 256), 256, 0, s>>>(st32, nu, nv, n_views, L.pitch_u,
-                                                          L.pitch_v);
+    const long long rows = (long long)L.pitch_v * n_views;
+    CTR_REQUIRE(rows < (1LL << 31), CTR_ERR_INVALID_ARGUMENT, "too many staged rows");
+    zero_border_kernel<<<(unsigned)rows, 64, 0, s>>>(st32, nu, nv, L.pitch_u, L.pitch_v);
     CTR_CHECK_LAUNCH();
+    // 16-bit storage: the filter records each row's max |q|, reduced per
+    // view into view_scale before quantisation
+    const bool fused_max = staged_dtype != CTR_F32;
+    float *row_max = nullptr;
+    if (fused_max) {
+        CTR_REQUIRE(view_scale, CTR_ERR_NULL_POINTER,
+                    "view_scale is required for 16-bit staged projections");
+        void *p = nullptr;
+        if ((rc = workspace_get(w, SLOT_VIEW_MAX, sizeof(float) * (size_t)nv * n_views, &p)))
+            return rc;
+        row_max = (float *)p;
+    }
     FilterJob J{};
     J.in = proj;
     J.width = nu;
@@ -1008,8 +1097,14 @@ int ctr_filter_projections(void *stream, ctr_workspace *ws, const float *proj, i
     J.staged = st32;
     J.pitch_u = L.pitch_u;
     J.pitch_v = L.pitch_v;
+    J.row_max = row_max;
     if ((rc = launch_filter_pairs(s, J, log2_exact(n)))) return rc;
-    return finish_staged(s, st32, nu, nv, n_views, staged, staged_dtype, view_scale);
+    if (fused_max) {
+        view_max_from_rows_kernel<<<n_views, 256, 0, s>>>(row_max, nv, n_views, layout,
+                                                           view_scale);
+        CTR_CHECK_LAUNCH();
+    }
+    return finish_staged(s, st32, nu, nv, n_views, staged, staged_dtype, view_scale, fused_max);
 }
 
 int ctr_stage_projections(void *stream, ctr_workspace *ws, const float *proj, int layout,
@@ -1041,7 +1136,7 @@ int ctr_stage_projections(void *stream, ctr_workspace *ws, const float *proj, in
     stage_plain_kernel<<<grid1d(total, 256), 256, 0, s>>>(proj, layout, nu, nv, n_views, st32,
                                                           L.pitch_u, L.pitch_v);
     CTR_CHECK_LAUNCH();
-    return finish_staged(s, st32, nu, nv, n_views, staged, staged_dtype, view_scale);
+    return finish_staged(s, st32, nu, nv, n_views, staged, staged_dtype, view_scale, false);
 }
 
 }  // extern "C"

@@ -140,9 +140,6 @@ int ctr_workspace_reserve(ctr_workspace *ws, int n_views, int max_fft_length)
     if ((rc = ctr::workspace_get(w, ctr::SLOT_GEOMETRY,
                                  ctr::kGeometryBytesPerView * (size_t)(n_views + 1), &p)))
         return rc;
-    if ((rc = ctr::workspace_get(w, ctr::SLOT_VIEW_MAX,
-                                 sizeof(unsigned) * (size_t)(n_views + 1), &p)))
-        return rc;
     return CTR_OK;
 }
 

@@ -80,11 +80,13 @@ int ctr_version(void);
 int ctr_synchronize(void);
 
 /* ---- workspace (scratch owned by the library) ------------------------------
- * Per-view geometry tables, FFT twiddles and per-view maxima live here so
- * that callers never see allocations inside hot loops.  Passing NULL to any
- * entry point uses a per-device default workspace.  A workspace grows on
- * demand (hipMalloc, synchronous); ctr_workspace_reserve pre-sizes it for
- * the given problem so that later calls allocate nothing.  No reference
+ * Per-view geometry tables, tile footprint tables, per-row maxima and the
+ * FP32 staging buffer of 16-bit filtering live here so that callers never
+ * see allocations inside hot loops.  Passing NULL to any entry point uses a
+ * per-device default workspace.  A workspace grows on demand (hipMalloc,
+ * synchronous) and never shrinks: one warm-up call of a problem leaves later
+ * calls of that size allocation-free (ctr_workspace_reserve pre-sizes the
+ * per-view geometry tables for n_views).  No reference
  * counterpart: the reference allocated textures per call
  * (FastFourierTransformGPU.cpp:15,74,115,138). */
 int ctr_workspace_create(ctr_workspace **out);
@@ -284,6 +286,35 @@ int ctr_render_ellipsoids(void *stream, ctr_workspace *ws,
  * No reference counterpart. */
 int ctr_poisson_log(void *stream, float *data, size_t count, double i0,
                     uint64_t seed);
+
+/* ---- formats either side of the path -------------------------------------- */
+/* Raw scanner frames -> float projections.  Replaces the per-view decode,
+ * resize and R16 upload of FBP3D::read_projections /
+ * read_projections_as_sinograms (FBP3D.cpp:74-176; the host reads the files).
+ * raw: device buffer of n_frames frames of file_w x file_h pixels, each of
+ * `channels` interleaved little-endian unsigned channels of
+ * bytes_per_channel (1 or 2) bytes.  Channel 0 is converted as an R8 / R16
+ * texture upload converts it (value / 255 or / 65535) and resampled to
+ * target_w x target_h per axis: box (area) filter when shrinking, linear
+ * between pixel centres when enlarging.  Frame f becomes view first_view + f
+ * of `out`, a (n_views, target_h, target_w) projection stack or a
+ * (target_h, n_views, target_w) sinogram stack per `layout`. */
+int ctr_decode_frames(void *stream, const void *raw, int file_w, int file_h, int channels,
+                      int bytes_per_channel, int n_frames, float *out, int layout,
+                      int target_w, int target_h, int n_views, int first_view);
+/* Minimum and maximum of `count` floats (NaN ignored; (0, 0) when none) into
+ * the device array min_max[2].  Replaces
+ * FBP3D::compute_{min,max}_value_of_volume (FBP3D.cpp:780-800). */
+int ctr_min_max(void *stream, ctr_workspace *ws, const float *data, size_t count,
+                float *min_max);
+/* data = (data - min) / (max - min) from the device array min_max[2] (0 when
+ * max == min).  Replaces FBP3D::normalize_histogram (FBP3D.cpp:802-826). */
+int ctr_normalize_range(void *stream, float *data, size_t count, const float *min_max);
+/* y-slices y0 .. y0 + count - 1 of a (Z, Y, X) float volume as unorm16
+ * (round(clamp(v, 0, 1) * 65535)), each slice Z rows of X values: the pixels
+ * FBP3D::write_volume (FBP3D.cpp:178-195) writes per slice file. */
+int ctr_slices_unorm16(void *stream, const float *volume, int X, int Y, int Z, int y0,
+                       int count, unsigned short *out);
 
 #ifdef __cplusplus
 }

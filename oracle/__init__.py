@@ -351,3 +351,64 @@ def render_ellipsoids(ellipsoids, shape, volume_width, threads=None):
     lib().orc_render_ellipsoids(_ptr(e), e.shape[0], X, Y, Z, volume_width,
                                 _ptr(vol), threads or threads_default())
     return vol
+
+
+# ---------------------------------------------------------------------------
+# Formats either side of the path (projection ingest, slice output).  The
+# reference does these with GraphicsCortex's Image class and GL conversions
+# (FBP3D.cpp:74-195,780-826), which are absent: the restatement below follows
+# the semantics the build documents (unorm conversion of channel 0, box
+# filter when shrinking, linear between pixel centres when enlarging,
+# unorm16 = round(clamp(v, 0, 1) * 65535)).  Parity with GraphicsCortex's
+# resize and its Image flags is unpinned; the functions are pinned by closed
+# forms in tests/test_oracle_formats.py.
+
+def resample_matrix(n_dst, n_src):
+    """(n_dst, n_src) weights of one axis: area coverage / scale when
+    shrinking, linear between pixel centres (clamped) when enlarging."""
+    m = np.zeros((n_dst, n_src), dtype=np.float64)
+    if n_dst <= n_src:
+        s = n_src / n_dst
+        for d in range(n_dst):
+            a, b = d * s, (d + 1) * s
+            for i in range(int(np.floor(a)), min(int(np.ceil(b)), n_src)):
+                m[d, i] = (min(b, i + 1) - max(a, i)) / s
+    else:
+        for d in range(n_dst):
+            x = min(max((d + 0.5) * n_src / n_dst - 0.5, 0.0), n_src - 1)
+            i0 = min(int(np.floor(x)), n_src - 1)
+            f = x - i0
+            if i0 + 1 < n_src and f > 0:
+                m[d, i0] = 1.0 - f
+                m[d, i0 + 1] = f
+            else:
+                m[d, i0] = 1.0
+    return m
+
+
+def decode_frame(raw, file_w, file_h, channels, bytes_per_channel, target_w, target_h):
+    """One raw frame (bytes) -> float64 (target_h, target_w): channel 0 as
+    unorm, resampled separably."""
+    raw = np.frombuffer(bytes(raw), dtype=np.uint8)
+    px = raw[:file_w * file_h * channels * bytes_per_channel].reshape(
+        file_h, file_w, channels, bytes_per_channel)[:, :, 0, :]
+    if bytes_per_channel == 1:
+        img = px[..., 0].astype(np.float64) / 255.0
+    else:
+        img = (px[..., 0].astype(np.float64) + 256.0 * px[..., 1]) / 65535.0
+    return resample_matrix(target_h, file_h) @ img @ resample_matrix(target_w, file_w).T
+
+
+def unorm16(v):
+    """round(clamp(v, 0, 1) * 65535) with the product in FP32 and
+    round-half-even, NaN -> 0."""
+    v = np.nan_to_num(np.asarray(v, dtype=np.float32), nan=0.0)
+    c = np.clip(v, np.float32(0), np.float32(1))
+    return np.rint(c * np.float32(65535)).astype(np.uint16)
+
+
+def normalize_range(v):
+    """(v - min) / (max - min), 0 when the range is empty (NaN ignored)."""
+    v = np.asarray(v, dtype=np.float64)
+    lo, hi = np.nanmin(v), np.nanmax(v)
+    return np.zeros_like(v) if hi <= lo else (v - lo) / (hi - lo)
