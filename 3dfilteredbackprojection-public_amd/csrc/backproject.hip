@@ -658,6 +658,14 @@ __device__ inline void lds_run_accumulate(float (&acc)[TY], const LdsRun &R, con
     }
 }
 
+// First view a workgroup visits: 0, so every tile accumulates its views in
+// the same order and results are reproducible bit for bit.
+__device__ inline int first_view(int n_views)
+{
+    (void)n_views;
+    return 0;
+}
+
 template <int DT, int TY>
 __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int bz,
                                             unsigned *lds)
@@ -699,12 +707,19 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
         out_scale = J.scale * m;
     }
 
+    // views are visited from v0 on, wrapping (v0 = 0: in order)
+    const int v0 = first_view(N);
+    auto vw = [&](int pos) {
+        const int v = pos + v0;
+        return v >= N ? v - N : v;
+    };
+
     // clear the row buffers (every LDS tap then reads a finite value), fill
     // the first rectangle table
     for (int t = threadIdx.x; t < L::kBufDwords / 2; t += 256)
         reinterpret_cast<uint4 *>(bufs)[t] = make_uint4(0u, 0u, 0u, 0u);
     if (threadIdx.x <= kRectChunk && (int)threadIdx.x < N)
-        rects[threadIdx.x] = tile_rect(J, foot, foot_stride, threadIdx.x, ja, jb, cv1, HMAX);
+        rects[threadIdx.x] = tile_rect(J, foot, foot_stride, vw(threadIdx.x), ja, jb, cv1, HMAX);
     __syncthreads();
 
     float acc[TY];
@@ -713,7 +728,8 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
 
     int4 cur = read_rect(rects, 0);
     if (cur.w == RECT_LDS)
-        stage_rect<L::kPitch>(vbase, view_bytes32, pitch_u, pitch_v, cur.x, cur.y, cur.z, bufs, wave, lane);
+        stage_rect<L::kPitch>(vbase + (size_t)vw(0) * view_bytes, view_bytes32, pitch_u, pitch_v,
+                              cur.x, cur.y, cur.z, bufs, wave, lane);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
 
@@ -726,14 +742,15 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
         if (n + 1 < N) {
             nxt = read_rect(rects, c + 1);
             if (nxt.w == RECT_LDS)
-                stage_rect<L::kPitch>(vbase + (size_t)(n + 1) * view_bytes, view_bytes32, pitch_u, pitch_v,
-                           nxt.x, nxt.y, nxt.z, bufs + ((n + 1) & 1) * L::kBufDwords, wave,
-                           lane);
+                stage_rect<L::kPitch>(vbase + (size_t)vw(n + 1) * view_bytes, view_bytes32,
+                                      pitch_u, pitch_v, nxt.x, nxt.y, nxt.z,
+                                      bufs + ((n + 1) & 1) * L::kBufDwords, wave, lane);
         }
         if (cur.w == RECT_LDS) {
+            const int view = vw(n);
             float vs = wscale;
-            if (J.view_scale) vs *= as_constant(J.view_scale)[n];
-            const ColumnView cv = column_view64(J.geom64 + (size_t)kGeom64 * n, i, k, j0, vs);
+            if (J.view_scale) vs *= as_constant(J.view_scale)[view];
+            const ColumnView cv = column_view64(J.geom64 + (size_t)kGeom64 * view, i, k, j0, vs);
             const unsigned buf_byte = (unsigned)(L::kRectBytes + (n & 1) * L::kBufDwords * 4);
             const LdsRun R = lds_run(cv, cur.x, cur.y, buf_byte);
             lds_run_accumulate<DT, TY>(acc, R, lds, lp4);
@@ -745,7 +762,7 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
             n0 += kRectChunk;
             if (threadIdx.x <= kRectChunk && n0 + (int)threadIdx.x < N)
                 rects[threadIdx.x] =
-                    tile_rect(J, foot, foot_stride, n0 + threadIdx.x, ja, jb, cv1, HMAX);
+                    tile_rect(J, foot, foot_stride, vw(n0 + threadIdx.x), ja, jb, cv1, HMAX);
             __syncthreads();
         }
         cur = nxt;

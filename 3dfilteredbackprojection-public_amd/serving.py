@@ -273,7 +273,10 @@ class Server:
                         r.checksum = sums[q]
                         if s.results is not None:
                             r.result = s.results[q]
+                    before = finished
                     finished += len(s.busy)
+                    if self.log and finished // 100 > before // 100:
+                        self.log(f"served {finished}/{total} at {t:.1f} s")
                     s.busy = []
                     s.last_used = t
                     progressed = True
