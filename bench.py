@@ -136,7 +136,9 @@ def serving_main(args):
     ctr = __graft_entry__.load()
     ctr.require_device()
     from ctrecon import serving
+    small, medium, large = (int(x) for x in args.max_slots.split(","))
     srv = serving.Server(budget_bytes=args.budget_gb * 1e9, dtype=ctr.F16, pack=args.pack,
+                         max_slots={"small": small, "medium": medium, "large": large},
                          pool=2, log=log)
     classes = ["small", "medium", "large"]
     t0 = time.time()
@@ -162,6 +164,7 @@ def serving_main(args):
         "requests": args.requests,
         "makespan_s": round(makespan, 2),
         "hbm_budget_gb": args.budget_gb,
+        "max_slots": args.max_slots,
         "slots": {c: sum(1 for x in srv.slots if x.cls == c) for c in classes},
         "slot_gb": round(srv.used / 1e9, 2),
         "per_class": s["per_class"],
@@ -184,6 +187,10 @@ def main():
     ap.add_argument("--rate", type=float, default=3.0)
     ap.add_argument("--pack", type=int, default=4)
     ap.add_argument("--budget-gb", type=float, default=200.0)
+    # one 1024^3 slot: a 1024^3 reconstruction fills the GPU for ~0.35 s, and
+    # several in flight delay every smaller request behind them
+    ap.add_argument("--max-slots", default="2,6,1",
+                    help="serving slots per class: small,medium,large")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
