@@ -472,6 +472,10 @@ struct LdsConfig<CTR_F32> {
 
 enum RectMode { RECT_SKIP = 0, RECT_LDS = 1, RECT_DIRECT = 2 };
 
+// A rectangle is int4 (u0, v0, rows, mode | cols << 8).
+__device__ inline int rect_mode(const int4 &r) { return r.w & 0xff; }
+__device__ inline int rect_cols(const int4 &r) { return r.w >> 8; }
+
 // LDS of the kernel: the rectangle table (kRectChunk + 1 int4), then two row
 // buffers.
 template <int DT>
@@ -531,7 +535,17 @@ __device__ inline int4 tile_rect(const BpJob &J, const float4 *foot, size_t foot
         mode = RECT_LDS;
     else
         mode = RECT_DIRECT;
-    return make_int4(u0, v0, v1 - v0 + 1, mode);
+    // start staged rows on a 32-texel (128-byte) boundary when the rectangle
+    // still fits the 64 staged columns, else on 16 texels: each row DMA then
+    // touches whole cache lines (the staged pitch is a multiple of 32 texels)
+    int ua = u0;
+    if (mode == RECT_LDS) {
+        if (u1 - (u0 & ~31) < kLdsCols)
+            ua = u0 & ~31;
+        else if (u1 - (u0 & ~15) < kLdsCols)
+            ua = u0 & ~15;
+    }
+    return make_int4(ua, v0, v1 - v0 + 1, mode | ((u1 - ua + 1) << 8));
 }
 
 __device__ inline int4 read_rect(const int4 *tab, int c)
@@ -551,9 +565,10 @@ __device__ inline int4 read_rect(const int4 *tab, int c)
 // work per row, and any address outside the view would read zero.
 template <int PITCH>
 __device__ inline void stage_rect(const char *view, unsigned view_bytes, int pitch_u,
-                                  int pitch_v, int u0, int v0, int h, unsigned *buf, int wave,
-                                  int lane)
+                                  int pitch_v, int u0, int v0, int h, int cols, unsigned *buf,
+                                  int wave, int lane)
 {
+    if (lane >= cols) return;  // only the rectangle's columns are staged
     const __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(view), (short)0, (int)view_bytes,
                                           0x00020000);
@@ -727,9 +742,9 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
     for (int t = 0; t < TY; ++t) acc[t] = 0.0f;
 
     int4 cur = read_rect(rects, 0);
-    if (cur.w == RECT_LDS)
+    if (rect_mode(cur) == RECT_LDS)
         stage_rect<L::kPitch>(vbase + (size_t)vw(0) * view_bytes, view_bytes32, pitch_u, pitch_v,
-                              cur.x, cur.y, cur.z, bufs, wave, lane);
+                              cur.x, cur.y, cur.z, rect_cols(cur), bufs, wave, lane);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
 
@@ -741,12 +756,12 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
         int4 nxt = make_int4(0, 0, 0, RECT_SKIP);
         if (n + 1 < N) {
             nxt = read_rect(rects, c + 1);
-            if (nxt.w == RECT_LDS)
+            if (rect_mode(nxt) == RECT_LDS)
                 stage_rect<L::kPitch>(vbase + (size_t)vw(n + 1) * view_bytes, view_bytes32,
-                                      pitch_u, pitch_v, nxt.x, nxt.y, nxt.z,
+                                      pitch_u, pitch_v, nxt.x, nxt.y, nxt.z, rect_cols(nxt),
                                       bufs + ((n + 1) & 1) * L::kBufDwords, wave, lane);
         }
-        if (cur.w == RECT_LDS) {
+        if (rect_mode(cur) == RECT_LDS) {
             const int view = vw(n);
             float vs = wscale;
             if (J.view_scale) vs *= as_constant(J.view_scale)[view];
@@ -755,7 +770,7 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
             const LdsRun R = lds_run(cv, cur.x, cur.y, buf_byte);
             lds_run_accumulate<DT, TY>(acc, R, lds, lp4);
         }
-        any_direct |= cur.w == RECT_DIRECT;
+        any_direct |= rect_mode(cur) == RECT_DIRECT;
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
         if (c + 1 == kRectChunk && n + 1 < N) {
@@ -775,7 +790,7 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
 #pragma unroll 1
         for (int n = 0; n < N; ++n) {
             const int mode = __builtin_amdgcn_readfirstlane(
-                tile_rect(J, foot, foot_stride, n, ja, jb, cv1, HMAX).w);
+                rect_mode(tile_rect(J, foot, foot_stride, n, ja, jb, cv1, HMAX)));
             if (mode != RECT_DIRECT) continue;
             float vs = wscale;
             if (J.view_scale) vs *= J.view_scale[n];

@@ -79,7 +79,9 @@ int workspace_get(ctr_workspace *ws, int slot, size_t bytes, void **ptr)
 StagedLayout staged_layout(int nu, int nv, int n_views, int dtype)
 {
     StagedLayout L;
-    L.pitch_u = (nu + 3 + 3) & ~3;  // one zero column before, two after
+    // one zero column before, at least two after; a multiple of 32 texels
+    // (128 bytes) so that rows start on cache-line boundaries
+    L.pitch_u = (nu + 3 + 31) & ~31;
     L.pitch_v = nv + 3;             // one zero row before, two after
     L.view = (size_t)L.pitch_u * (size_t)L.pitch_v;
     size_t texel = 4;               // fp32 texel, or a 16-bit pair

@@ -171,7 +171,9 @@ int ctr_filter_projections(void *stream, ctr_workspace *ws, const float *proj,
  * View-major, zero-bordered copy of the filtered projections that the
  * backprojector samples.  For CTR_F32 texel (iu, iv, n) is
  *   staged[(n * pitch_v + iv + 1) * pitch_u + iu + 1],
- * with one zero column/row before and two after the data.  For CTR_F16 and
+ * with one zero column/row before the data, two zero rows and at least two
+ * zero columns after it; pitch_u is a multiple of 32 texels (128-byte rows).
+ * For CTR_F16 and
  * CTR_BF16 each 32-bit texel holds the pair (q[iu], q[iu+1]) / view_scale[n]
  * so one aligned load feeds both horizontal taps.  No reference counterpart
  * (the reference sampled its Texture2DArray directly, FBP3D.cpp:543). */
