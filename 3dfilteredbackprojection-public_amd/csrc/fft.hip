@@ -31,6 +31,29 @@ __global__ void init_twiddles_kernel()
     }
 }
 
+// Filter gains of the unshifted spectrum, per filter: entry n + k holds the
+// gain of index k of an n-point row (n a power of two <= 8192), the same
+// value unshifted_gain computes; filled once per device next to the
+// twiddles so that the row filters read a table instead of evaluating the
+// FP64 sine of the Shepp-Logan / cosine gains per sample.
+__device__ float g_gain[3][2 * kMaxN];
+
+__device__ inline float filter_g(int filter, double x);
+__device__ inline float unshifted_gain(int filter, int k, int n);
+
+__global__ void init_gains_kernel()
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // over 3 * 2 * kMaxN
+    if (idx >= 3 * 2 * kMaxN) return;
+    const int f = idx / (2 * kMaxN), e = idx % (2 * kMaxN);
+    if (e == 0) {
+        g_gain[f][0] = 0.0f;
+        return;
+    }
+    const int n = 1 << (31 - __clz(e));  // largest power of two <= e
+    g_gain[f][e] = unshifted_gain(f, e - n, n);
+}
+
 static std::mutex g_tw_lock;
 static bool g_tw_ready[64] = {false};
 
@@ -43,6 +66,8 @@ static int ensure_twiddles()
     std::lock_guard<std::mutex> g(g_tw_lock);
     if (!g_tw_ready[dev]) {
         init_twiddles_kernel<<<kMaxN / 2 / 256, 256, 0, 0>>>();
+        CTR_CHECK_LAUNCH();
+        init_gains_kernel<<<3 * 2 * kMaxN / 256, 256, 0, 0>>>();
         CTR_CHECK_LAUNCH();
         CTR_TRY_HIP(hipDeviceSynchronize());
         g_tw_ready[dev] = true;
@@ -242,6 +267,15 @@ __global__ __launch_bounds__(256) void view_max_from_rows_kernel(const float *ro
     }
 }
 
+// FDK cosine weight SDD / sqrt(SDD^2 + u^2 + v^2) of detector column e for a
+// row with vv = SDD^2 + v^2: FP32 reciprocal square root (about 1e-7
+// relative error on a smooth weight; the row terms come from FP64).
+__device__ inline float fdk_weight_at(float sdd, float vv, float cu, float delta, int e)
+{
+    const float u = ((float)e - cu) * delta;
+    return sdd * rsqrtf(fmaf(u, u, vv));
+}
+
 __device__ inline void row_coords(const FilterJob &J, int r, int &view, int &iv)
 {
     if (J.layout == CTR_LAYOUT_PROJECTIONS) {
@@ -275,6 +309,8 @@ __global__ __launch_bounds__(kFftThreads) void filter_pairs_kernel(FilterJob J)
     }
     const float *a_row = J.in + (size_t)r0 * J.width;
     const float *b_row = J.in + (size_t)r1 * J.width;
+    const float sdd_f = (float)sqrt(J.sdd2), vv0_f = (float)vv0, vv1_f = (float)vv1;
+    const float cu_f = (float)cu, delta_f = (float)J.delta;
 
     load_twiddles<LOG2N>(T);
     for (int e = threadIdx.x; e < N; e += blockDim.x) {
@@ -283,10 +319,8 @@ __global__ __launch_bounds__(kFftThreads) void filter_pairs_kernel(FilterJob J)
             a = a_row[e];
             if (has1) b = b_row[e];
             if (J.fdk) {
-                double u = (e - cu) * J.delta;
-                double sdd = sqrt(J.sdd2);
-                a = (float)((double)a * (sdd / sqrt(vv0 + u * u)));
-                b = (float)((double)b * (sdd / sqrt(vv1 + u * u)));
+                a *= fdk_weight_at(sdd_f, vv0_f, cu_f, delta_f, e);
+                b *= fdk_weight_at(sdd_f, vv1_f, cu_f, delta_f, e);
             }
         }
         L[bitrev<LOG2N>(e)] = make_float2(a, b);
@@ -300,7 +334,7 @@ __global__ __launch_bounds__(kFftThreads) void filter_pairs_kernel(FilterJob J)
     for (int p = 0; p < PER; ++p) {
         int k = threadIdx.x + p * kFftThreads;
         if (k < N) {
-            float g = unshifted_gain(J.filter, k, N);
+            float g = g_gain[J.filter][N + k];
             float2 v = L[k];
             reg[p] = make_float2(v.x * g, v.y * g);
         }
@@ -508,6 +542,8 @@ __global__ __launch_bounds__((1 << LOG2N) / 8) void filter_stockham_kernel(Filte
     }
     const float *a_row = J.in + (size_t)r0 * J.width;
     const float *b_row = J.in + (size_t)r1 * J.width;
+    const float sdd_f = (float)sdd, vv0_f = (float)vv0, vv1_f = (float)vv1;
+    const float cu_f = (float)cu, delta_f = (float)J.delta;
 
     // forward pass 0 (radix 8, span 1) straight from the detector rows
     float2 v[8];
@@ -519,9 +555,8 @@ __global__ __launch_bounds__((1 << LOG2N) / 8) void filter_stockham_kernel(Filte
             a = a_row[e];
             if (has1) b = b_row[e];
             if (J.fdk) {
-                double u = (e - cu) * J.delta;
-                a = (float)((double)a * (sdd / sqrt(vv0 + u * u)));
-                b = (float)((double)b * (sdd / sqrt(vv1 + u * u)));
+                a *= fdk_weight_at(sdd_f, vv0_f, cu_f, delta_f, e);
+                b *= fdk_weight_at(sdd_f, vv1_f, cu_f, delta_f, e);
             }
         }
         v[r] = make_float2(a, b);
@@ -538,7 +573,7 @@ __global__ __launch_bounds__((1 << LOG2N) / 8) void filter_stockham_kernel(Filte
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
         const int e = tid + r * T;
-        const float g = unshifted_gain(J.filter, e, N);
+        const float g = g_gain[J.filter][N + e];
         const float2 x = L[lpad(e)];
         v[r] = make_float2(x.x * g, x.y * g);
     }
@@ -614,7 +649,7 @@ static int launch_filter_pairs(hipStream_t s, const FilterJob &J, int log2n)
 // ---------------------------------------------------------------------------
 // Zeroes the border texels of an fp32 staged buffer: rows 0, nv+1, nv+2 of
 // every view entirely, columns 0 and nu+1 .. pitch_u-1 of the others.  One
-// 64-thread workgroup per staged row (pitch_u - nu <= 6 border columns).
+// 64-thread workgroup per staged row (pitch_u - nu <= 35 border columns).
 __global__ __launch_bounds__(64) void zero_border_kernel(float *st, int nu, int nv, int pitch_u,
                                                          int pitch_v)
 {
@@ -686,7 +721,7 @@ __device__ inline unsigned short f32_to_f16_bits(float f)
 // fp32 staged -> 16-bit pair texels: texel c of a row holds
 // (q[c], q[c+1]) / view_scale, so one aligned 32-bit load gives both
 // horizontal bilinear taps.  One workgroup per staged row, four texels per
-// thread (pitch_u is a multiple of 4).  A view whose max is 0 gets scale 1.
+// thread (pitch_u is a multiple of 32).  A view whose max is 0 gets scale 1.
 __device__ inline unsigned pack_pair(float a, float b, int bf16)
 {
     return bf16 ? (unsigned)f32_to_bf16_rne(a) | ((unsigned)f32_to_bf16_rne(b) << 16)
