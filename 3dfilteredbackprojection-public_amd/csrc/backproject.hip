@@ -416,10 +416,13 @@ __global__ __launch_bounds__(256) void backproject_kernel(BpJob J)
 //   fp32 / bf16 -- v FMA, fract, cvt, mad24, the taps (two ds_read2_b32 for
 //     fp32, one for 16-bit pairs), two lerps and two FMAs into the
 //     accumulator (10 VALU for fp32, +4 bit ops for bf16 pairs);
+//   the fp32 and fp16 loops are software-pipelined: the LDS reads of the
+//   next batch of voxels are issued before the current batch is combined;
 //   fp16 -- v FMA, fract, cvt, mad24, one ds_read2_b32 of the (u, u + 1)
-//     pairs of rows I and I + 1, the weight to f16, a packed f16 difference
-//     and lerp, and v_dot2_f32_f16 of the lerped pair with the column's f16
-//     (1 - wu, wu) weights into the FP32 accumulator (8 VALU).  Those f16
+//     pairs of rows I and I + 1, the weight to f16 (one v_cvt_pk_f16_f32 per
+//     two voxels), a packed f16 difference and lerp, and v_dot2_f32_f16 of
+//     the lerped pair with the column's f16 (1 - wu, wu) weights into the
+//     FP32 accumulator (7.5 VALU).  Those f16
 //     weights hold the distance weight times view_scale / max(view_scale)
 //     (at most (SOD / (SOD - r))^2, r the volume's half diagonal; the host
 //     keeps this path for geometries where that stays far below the f16
@@ -456,6 +459,16 @@ __global__ __launch_bounds__(256) void backproject_kernel(BpJob J)
 #endif
 #ifndef CTR_LDS_TY
 #define CTR_LDS_TY 32
+#endif
+// Voxels per software-pipelined batch of LDS reads (fp32 / packed fp16).
+// Measured on 512^3 x 720 and 1024^3 x 1200 (GUPS): fp32 batch 4 2767 /
+// 3254, batch 2 2792 / 3389, batch 8 2625 / 3071; packed fp16 without
+// pipelining 3343 / 3751, batch 2 3556 / 4146, batch 4 3487 / 4120.
+#ifndef CTR_F32_BATCH
+#define CTR_F32_BATCH 2
+#endif
+#ifndef CTR_F16_BATCH
+#define CTR_F16_BATCH 2
 #endif
 constexpr int kLdsCols = 64;    // texels staged per row (one per lane)
 constexpr int kRectChunk = 128; // views per refill of the rectangle table
@@ -613,26 +626,56 @@ __device__ inline void lds_run_accumulate(float (&acc)[TY], const LdsRun &R, con
 {
     if constexpr (DT == kF16Packed) {
         const half2_t W = {(_Float16)R.cL, (_Float16)R.cR};
+        // software-pipelined like the fp32 branch: the pair reads of batch
+        // b + 1 are in flight while batch b is interpolated; the row weights
+        // of two voxels share one packed conversion
+        constexpr int kB = CTR_F16_BATCH;
+        static_assert(kB % 2 == 0 && TY % kB == 0, "batch");
+        unsigned ta[2][kB], tb[2][kB];
+        float tw[2][kB];
+        auto issue = [&](int set, int t0) {
 #pragma unroll
-        for (int t = 0; t < TY; ++t) {
-            const float x = fmaf(R.dv, (float)t, R.f);
-            const float wv = __builtin_amdgcn_fractf(x);
-            const unsigned addr = __umul24((unsigned)x, lp4) + R.base;
-            const unsigned *P =
-                reinterpret_cast<const unsigned *>(reinterpret_cast<const char *>(lds) + addr);
-            const half2_t a = __builtin_bit_cast(half2_t, P[0]);
-            const half2_t b = __builtin_bit_cast(half2_t, P[LdsConfig<DT>::kPitch]);
-            const _Float16 w16 = (_Float16)wv;
-            const half2_t w = {w16, w16};
-            const half2_t lr = __builtin_elementwise_fma(w, b - a, a);
-            acc[t] = __builtin_amdgcn_fdot2(__builtin_bit_cast(fp16x2_t, lr),
-                                            __builtin_bit_cast(fp16x2_t, W), acc[t], false);
+            for (int q = 0; q < kB; ++q) {
+                const float x = fmaf(R.dv, (float)(t0 + q), R.f);
+                tw[set][q] = __builtin_amdgcn_fractf(x);
+                const unsigned *P = reinterpret_cast<const unsigned *>(
+                    reinterpret_cast<const char *>(lds) + __umul24((unsigned)x, lp4) + R.base);
+                ta[set][q] = P[0];
+                tb[set][q] = P[LdsConfig<DT>::kPitch];
+            }
+        };
+        issue(0, 0);
+#pragma unroll
+        for (int bt = 0; bt < TY / kB; ++bt) {
+            if (bt + 1 < TY / kB) issue((bt + 1) & 1, (bt + 1) * kB);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < kB; q += 2) {
+                // one v_cvt_pk_f16_f32 for both weights: the empty asm keeps
+                // the compiler from splitting it into two scalar conversions
+                unsigned wbits = __builtin_bit_cast(
+                    unsigned, half2_t{(_Float16)tw[bt & 1][q], (_Float16)tw[bt & 1][q + 1]});
+                asm volatile("" : "+v"(wbits));
+                const half2_t wp = __builtin_bit_cast(half2_t, wbits);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const half2_t a = __builtin_bit_cast(half2_t, ta[bt & 1][q + h]);
+                    const half2_t b = __builtin_bit_cast(half2_t, tb[bt & 1][q + h]);
+                    const half2_t w = h ? half2_t{wp.y, wp.y} : half2_t{wp.x, wp.x};
+                    const half2_t lr = __builtin_elementwise_fma(w, b - a, a);
+                    acc[bt * kB + q + h] =
+                        __builtin_amdgcn_fdot2(__builtin_bit_cast(fp16x2_t, lr),
+                                               __builtin_bit_cast(fp16x2_t, W),
+                                               acc[bt * kB + q + h], false);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
         }
     } else if constexpr (DT == CTR_F32) {
         // fp32 reads four taps per voxel: software-pipelined in batches of
         // kB voxels (taps of batch b + 1 in flight while batch b is
         // interpolated), the scheduling barriers keep that order
-        constexpr int kB = 4;
+        constexpr int kB = CTR_F32_BATCH;
         float2 ta[2][kB], tb[2][kB];
         float tw[2][kB];
         auto issue = [&](int set, int t0) {

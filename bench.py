@@ -200,12 +200,16 @@ def main():
         serving_main(args)
         return
     if args.mode == "copy-probe":
-        # a plain device copy of 1 GiB, the reference for scaling the HBM
-        # byte counters (FETCH_SIZE / WRITE_SIZE) of the profiled kernels
+        # a plain elementwise kernel that reads 1 GiB and writes 1 GiB, the
+        # reference for scaling the HBM byte counters (FETCH_SIZE /
+        # WRITE_SIZE) of the profiled kernels.  (A clone goes through the
+        # runtime's copy path instead, whose counter readings for the same
+        # 1 GiB varied between runs by a quarter.)
         import torch
         x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
         for _ in range(3):
-            y = x.clone()
+            torch.neg(x, out=y)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": 4 << 28, "checksum": float(y[:16].sum())}))
         return
