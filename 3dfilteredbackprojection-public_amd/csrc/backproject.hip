@@ -394,7 +394,7 @@ __global__ __launch_bounds__(256) void backproject_kernel(BpJob J)
 // LDS-staged kernel.
 //
 // Per view, the block's detector footprint -- the bounding rectangle of the
-// projections of its 16 x 16 x TY voxel tile, at most 64 texels wide and
+// projections of its 16 x 16 x TY voxel tile, at most CMAX texels wide and
 // HMAX rows high -- is copied from the staged projections into LDS with
 // global_load_lds (one wave instruction per row, addresses in SGPRs, no VGPR
 // round trip).  Two row buffers alternate: the rows of view n + 1 are in
@@ -429,33 +429,46 @@ __global__ __launch_bounds__(256) void backproject_kernel(BpJob J)
 //     range), and the run is rescaled by scale * max(view_scale) at the
 //     store.
 // ---------------------------------------------------------------------------
-// LDS geometry per staged format: row pitch in dwords and rows per buffer.
-// The pitch is 64 staged texels plus padding that staggers the banks of
+// LDS geometry per staged format: staged columns, row pitch in dwords and
+// rows per buffer.  A 16 x 16 column tile spans at most 44 detector columns
+// on the 512^3 scan and fewer than 48 on 99.8% of the 1440^3 scan's
+// (tile, view) pairs, so a row holds 48 texels (the rest take the
+// direct-tap pass).  The pitch adds padding that staggers the banks of
 // neighbouring rows.  The lanes of a wave read taps from an 8 x 8 patch of
 // columns, i.e. from texels within about +-10 columns and +-2 rows of each
 // other; with 32 banks a row shift of s banks makes (row + dr, col + du)
 // collide when s dr + du = 0 mod 32: s = 1 (pitch 65) collides for every
-// (1, -1) neighbour, s = 5 (69) only for |du| >= 5 pairs two rows apart,
-// s = 11 (75) only for |du| >= 10.  Two buffers of `rows` rows bound the
-// workgroups per CU (160 KB of LDS): 84 rows at pitch 75 give three, 70 rows
-// at pitch 69 four.  Measured on 512^3 x 720 (GUPS, pitch/rows):
-//   fp32  65/96 1712, 75/84 2662, 75/64 2598, 69/70 2386
-//   fp16  65/96 2664, 75/84 2921, 72/66 3065, 69/70 3243
+// (1, -1) neighbour, s = 5 (69) for |du| >= 5 pairs two rows apart, s = 11
+// (75) and s = -11 (53) only for |du| >= 10 or three rows apart.  Two
+// buffers of `rows` rows bound the workgroups per CU (160 KB of LDS): fp32
+// at 53 x 84 fits four (its 120 VGPRs allow no more), 16-bit pairs at
+// 51 x 72 five.  Measured (GUPS, cols/pitch/rows; 512^3 x 720, 1024^3 x 1200,
+// 1440^3 x 1440):
+//   fp32  64/65/96 1712 (512^3 only), 64/75/84 2825 3399 3480,
+//         48/53/84 2909 3637 3795, 48/51/84 2805 3625 3804
+//   fp16  64/65/96 2664 (512^3 only), 64/69/70 3618 4149 4009,
+//         48/53/70 3738 4296 4259, 48/51/72 3760 4302 4351
 // fp32 reads four taps per voxel and is the most sensitive to conflicts;
-// 16-bit pairs read two and profit more from the fourth workgroup.  Every
-// 32-voxel footprint of the 512^3 scan fits 84 rows, 99.9% fit 70 (98% of
+// 16-bit pairs read two and profit more from the fifth workgroup.  Every
+// 32-voxel footprint of the 512^3 scan fits 72 rows bar 0.01% (98.9% of
 // the 1440^3 scan); larger ones take the direct-tap pass.
+#ifndef CTR_LDS_COLS32
+#define CTR_LDS_COLS32 48
+#endif
 #ifndef CTR_LDS_PITCH32
-#define CTR_LDS_PITCH32 75
+#define CTR_LDS_PITCH32 53
 #endif
 #ifndef CTR_LDS_ROWS32
 #define CTR_LDS_ROWS32 84
 #endif
+#ifndef CTR_LDS_COLS16
+#define CTR_LDS_COLS16 48
+#endif
 #ifndef CTR_LDS_PITCH16
-#define CTR_LDS_PITCH16 69
+#define CTR_LDS_PITCH16 51
 #endif
 #ifndef CTR_LDS_ROWS16
-#define CTR_LDS_ROWS16 70
+#define CTR_LDS_ROWS16 72
 #endif
 #ifndef CTR_LDS_TY
 #define CTR_LDS_TY 32
@@ -470,17 +483,16 @@ __global__ __launch_bounds__(256) void backproject_kernel(BpJob J)
 #ifndef CTR_F16_BATCH
 #define CTR_F16_BATCH 2
 #endif
-constexpr int kLdsCols = 64;    // texels staged per row (one per lane)
 constexpr int kRectChunk = 128; // views per refill of the rectangle table
 constexpr int kF16Packed = 16;  // tile variant: fp16 pairs with packed f16 arithmetic
 
 template <int DT>
 struct LdsConfig {
-    static constexpr int kPitch = CTR_LDS_PITCH16, kRows = CTR_LDS_ROWS16;
+    static constexpr int kCols = CTR_LDS_COLS16, kPitch = CTR_LDS_PITCH16, kRows = CTR_LDS_ROWS16;
 };
 template <>
 struct LdsConfig<CTR_F32> {
-    static constexpr int kPitch = CTR_LDS_PITCH32, kRows = CTR_LDS_ROWS32;
+    static constexpr int kCols = CTR_LDS_COLS32, kPitch = CTR_LDS_PITCH32, kRows = CTR_LDS_ROWS32;
 };
 
 enum RectMode { RECT_SKIP = 0, RECT_LDS = 1, RECT_DIRECT = 2 };
@@ -493,12 +505,13 @@ __device__ inline int rect_cols(const int4 &r) { return r.w >> 8; }
 // buffers.
 template <int DT>
 struct LdsLayout {
-    static constexpr int kPitch = LdsConfig<DT>::kPitch, kRows = LdsConfig<DT>::kRows;
+    static constexpr int kCols = LdsConfig<DT>::kCols, kPitch = LdsConfig<DT>::kPitch,
+                         kRows = LdsConfig<DT>::kRows;
     static constexpr int kRectBytes = (kRectChunk + 1) * 16;
     static constexpr int kBufDwords = kRows * kPitch;
     static constexpr int kDwords = kRectBytes / 4 + 2 * kBufDwords;
     static_assert((2 * kBufDwords) % 4 == 0, "buffers are cleared as uint4");
-    static_assert(kPitch >= kLdsCols + 1 && kPitch + 1 <= 255, "ds_read2 offsets");
+    static_assert(kCols <= 64 && kPitch >= kCols + 1 && kPitch + 1 <= 255, "ds_read2 offsets");
 };
 
 // Taps at LDS byte address `addr`: rows I (top) and I + 1 (bot), columns
@@ -529,7 +542,7 @@ __device__ inline void lds_taps(const unsigned *L, unsigned addr, float2 &top, f
 // conversions stay exact; a clamped rectangle is still classified correctly
 // because the clamp margin exceeds both buffer limits.
 __device__ inline int4 tile_rect(const BpJob &J, const float4 *foot, size_t foot_stride, int n,
-                                 float ja, float jb, float cv1, int hmax)
+                                 float ja, float jb, float cv1, int hmax, int cmax)
 {
     const float4 fp = foot[(size_t)n * foot_stride];
     const float vj = J.geom[3 * n].w;
@@ -544,18 +557,18 @@ __device__ inline int4 tile_rect(const BpJob &J, const float4 *foot, size_t foot
     int mode;
     if (u1 <= 0 || v1 <= 0 || u0 >= J.nu + 1 || v0 >= J.nv + 1)
         mode = RECT_SKIP;
-    else if (u1 - u0 < kLdsCols && v1 - v0 < hmax)
+    else if (u1 - u0 < cmax && v1 - v0 < hmax)
         mode = RECT_LDS;
     else
         mode = RECT_DIRECT;
     // start staged rows on a 32-texel (128-byte) boundary when the rectangle
-    // still fits the 64 staged columns, else on 16 texels: each row DMA then
+    // still fits the cmax staged columns, else on 16 texels: each row DMA then
     // touches whole cache lines (the staged pitch is a multiple of 32 texels)
     int ua = u0;
     if (mode == RECT_LDS) {
-        if (u1 - (u0 & ~31) < kLdsCols)
+        if (u1 - (u0 & ~31) < cmax)
             ua = u0 & ~31;
-        else if (u1 - (u0 & ~15) < kLdsCols)
+        else if (u1 - (u0 & ~15) < cmax)
             ua = u0 & ~15;
     }
     return make_int4(ua, v0, v1 - v0 + 1, mode | ((u1 - ua + 1) << 8));
@@ -569,13 +582,13 @@ __device__ inline int4 read_rect(const int4 *tab, int c)
 }
 
 // Rows v0 .. v0 + h - 1 of one staged view into an LDS row buffer, row r by
-// wave r mod 4, 64 columns from u0.  Rows outside the view are read from its
-// first or last row, columns outside it from its last column: all zero in
-// every texel format (column 0 is not -- a 16-bit pair texel there holds
-// (0, q[1])).  The loads go through a
-// buffer resource spanning exactly the view (raw 32-bit format), with the
-// row offset in an SGPR and the lane's column in the VGPR offset: no VALU
-// work per row, and any address outside the view would read zero.
+// wave r mod 4, `cols` columns from u0 (one per lane).  Rows outside the
+// view are read from its first or last row, columns outside it from its
+// last column: all zero in every texel format (column 0 is not -- a 16-bit
+// pair texel there holds (0, q[1])).  The loads go through a buffer
+// resource spanning exactly the view (raw 32-bit format), with the row
+// offset in an SGPR and the lane's column in the VGPR offset: no VALU work
+// per row, and any address outside the view would read zero.
 template <int PITCH>
 __device__ inline void stage_rect(const char *view, unsigned view_bytes, int pitch_u,
                                   int pitch_v, int u0, int v0, int h, int cols, unsigned *buf,
@@ -603,7 +616,7 @@ struct LdsRun {
     unsigned base;  // LDS byte address of (row 0, column of the left tap)
 };
 
-__device__ inline LdsRun lds_run(const ColumnView &c, int u0, int v0, unsigned buf_byte)
+__device__ inline LdsRun lds_run(const ColumnView &c, int u0, int v0, unsigned buf_byte, int cmax)
 {
     LdsRun R;
     const double su = c.su - (double)u0;
@@ -613,7 +626,7 @@ __device__ inline LdsRun lds_run(const ColumnView &c, int u0, int v0, unsigned b
     R.cL = c.w2 - R.cR;
     R.f = (float)(c.sv0 - (double)v0);
     R.dv = c.dv;
-    R.base = buf_byte + (unsigned)min(max((int)sfl, 0), kLdsCols - 2) * 4u;
+    R.base = buf_byte + (unsigned)min(max((int)sfl, 0), cmax - 2) * 4u;
     return R;
 }
 
@@ -729,7 +742,7 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
                                             unsigned *lds)
 {
     using L = LdsLayout<DT>;
-    constexpr int HMAX = L::kRows;
+    constexpr int HMAX = L::kRows, CMAX = L::kCols;
     constexpr int kTapDT = DT == kF16Packed ? CTR_F16 : DT;  // staged texel format
     int4 *rects = reinterpret_cast<int4 *>(lds);
     unsigned *bufs = lds + L::kRectBytes / 4;
@@ -777,7 +790,7 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
     for (int t = threadIdx.x; t < L::kBufDwords / 2; t += 256)
         reinterpret_cast<uint4 *>(bufs)[t] = make_uint4(0u, 0u, 0u, 0u);
     if (threadIdx.x <= kRectChunk && (int)threadIdx.x < N)
-        rects[threadIdx.x] = tile_rect(J, foot, foot_stride, vw(threadIdx.x), ja, jb, cv1, HMAX);
+        rects[threadIdx.x] = tile_rect(J, foot, foot_stride, vw(threadIdx.x), ja, jb, cv1, HMAX, CMAX);
     __syncthreads();
 
     float acc[TY];
@@ -810,7 +823,7 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
             if (J.view_scale) vs *= as_constant(J.view_scale)[view];
             const ColumnView cv = column_view64(J.geom64 + (size_t)kGeom64 * view, i, k, j0, vs);
             const unsigned buf_byte = (unsigned)(L::kRectBytes + (n & 1) * L::kBufDwords * 4);
-            const LdsRun R = lds_run(cv, cur.x, cur.y, buf_byte);
+            const LdsRun R = lds_run(cv, cur.x, cur.y, buf_byte, CMAX);
             lds_run_accumulate<DT, TY>(acc, R, lds, lp4);
         }
         any_direct |= rect_mode(cur) == RECT_DIRECT;
@@ -820,7 +833,7 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
             n0 += kRectChunk;
             if (threadIdx.x <= kRectChunk && n0 + (int)threadIdx.x < N)
                 rects[threadIdx.x] =
-                    tile_rect(J, foot, foot_stride, vw(n0 + threadIdx.x), ja, jb, cv1, HMAX);
+                    tile_rect(J, foot, foot_stride, vw(n0 + threadIdx.x), ja, jb, cv1, HMAX, CMAX);
             __syncthreads();
         }
         cur = nxt;
@@ -833,7 +846,7 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
 #pragma unroll 1
         for (int n = 0; n < N; ++n) {
             const int mode = __builtin_amdgcn_readfirstlane(
-                rect_mode(tile_rect(J, foot, foot_stride, n, ja, jb, cv1, HMAX)));
+                rect_mode(tile_rect(J, foot, foot_stride, n, ja, jb, cv1, HMAX, CMAX)));
             if (mode != RECT_DIRECT) continue;
             float vs = wscale;
             if (J.view_scale) vs *= J.view_scale[n];

@@ -288,7 +288,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(ach16, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(ach16 / HBM_PEAK_GBPS, 4),
                      "traffic": traffic,
-                     "kernel": "backproject_lds_kernel<fp16 packed, 32> (LDS pitch 69, 70 rows)",
+                     "kernel": "backproject_lds_kernel<fp16 packed, 32> (LDS 48 columns, pitch 51, 72 rows)",
                      "avg_ms": round(bp_ms, 3)},
         "cpu_baseline": cpu,
     }
