@@ -43,3 +43,25 @@ def rel_rmse(got, ref):
     got = np.asarray(got, dtype=np.float64)
     ref = np.asarray(ref, dtype=np.float64)
     return float(np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-300))
+
+
+def error_metrics(got, ref):
+    """rel RMSE, max-abs, max-rel (over max |ref|) and PSNR = 20 log10(max |ref|
+    / RMSE) of `got` against `ref`."""
+    import math
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    d = got - ref
+    rmse = math.sqrt(float(np.mean(d * d)))
+    peak = float(np.abs(ref).max())
+    return {
+        "rel_rmse": float(np.linalg.norm(d) / np.linalg.norm(ref)),
+        "max_abs": float(np.abs(d).max()),
+        "max_rel": float(np.abs(d).max()) / peak,
+        "psnr_db": 20.0 * math.log10(peak / rmse) if rmse > 0 else float("inf"),
+    }
+
+
+def format_metrics(m):
+    return (f"rel RMSE {m['rel_rmse']:.2e}, max-abs {m['max_abs']:.2e}, "
+            f"max-rel {m['max_rel']:.2e}, PSNR {m['psnr_db']:.1f} dB")

@@ -12,7 +12,7 @@ import math
 import numpy as np
 import pytest
 
-from conftest import rel_max, rel_rmse
+from conftest import error_metrics, format_metrics, rel_max, rel_rmse
 
 pytestmark = pytest.mark.gpu
 
@@ -122,12 +122,12 @@ def test_c3_storage_forms_vs_oracle(ctr, orc, dev):
     ref = orc.backproject_cone_region(filtered, g.source_distance, g.detector_distance,
                                       g.detector_width, g.volume_width, 1.0, 0.0, scale,
                                       (g.X, g.Y, g.Z), box)
-    errs = {dt: (rel_max(got[dt], ref), rel_rmse(got[dt], ref)) for dt in got}
-    print("[c3] max-rel / rel-RMSE vs FP64 oracle: " +
-          ", ".join(f"dtype {dt}: {a:.2e} / {b:.2e}" for dt, (a, b) in errs.items()), flush=True)
-    assert errs[ctr.F32][0] < 1e-4
-    assert errs[ctr.F16][1] < 1e-3
-    assert errs[ctr.BF16][1] < 1e-3
+    errs = {dt: error_metrics(got[dt], ref) for dt in got}
+    for name, dt in (("fp32", ctr.F32), ("fp16", ctr.F16), ("bf16", ctr.BF16)):
+        print(f"[accuracy] fdk_1440 centre 64^3 {name}: {format_metrics(errs[dt])}", flush=True)
+    assert errs[ctr.F32]["max_rel"] < 1e-4
+    assert errs[ctr.F16]["rel_rmse"] < 1e-3
+    assert errs[ctr.BF16]["rel_rmse"] < 1e-3
 
 
 def test_full_size_constant_projection_closed_form(ctr, dev):
