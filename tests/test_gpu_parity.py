@@ -207,3 +207,23 @@ def test_error_paths(ctr, dev):
         ctr.filter_rows(torch.zeros((2, 100), device=dev), 0, 96)
     with pytest.raises(ctr.CtrError):
         ctr.filter_rows(torch.zeros((2, 100), device=dev), 7, 128)
+
+
+@pytest.mark.parametrize("X,Y,Z,n_views", [(37, 45, 29, 130), (33, 64, 18, 1)])
+def test_lds_kernel_edge_shapes(ctr, orc, dev, monkeypatch, X, Y, Z, n_views):
+    """The LDS-staged kernel on ragged tiles (X, Z not multiples of 16), a
+    partial last y-run (Y not a multiple of 32), a view count that crosses
+    the 128-view refill of the rectangle table, and a single view; every
+    storage form against the FP64 oracle of the FP32 data."""
+    g = _cone(ctr, X=X, Y=Y, Z=Z, n_views=n_views, rotation_offset=0.2)
+    rng = np.random.default_rng(X * Y + n_views)
+    q = rng.standard_normal((g.n_views, g.nv, g.nu)).astype(np.float32)
+    ref = _fdk_ref(orc, g, q, 0.01)
+    monkeypatch.setenv("CTR_BACKPROJECT_KERNEL", "lds")
+    for dt, tol in ((ctr.F32, 1e-4), (ctr.F16, 1e-3), (ctr.BF16, 4e-3)):
+        staged, vs = ctr.stage_projections(_t(dev, q), g, dt)
+        got = ctr.backproject(staged, g, dt, view_scale=vs, scale=0.01).cpu().numpy()
+        if dt == ctr.F32:
+            assert rel_max(got, ref) < tol, dt
+        else:
+            assert rel_rmse(got, ref) < tol, dt
