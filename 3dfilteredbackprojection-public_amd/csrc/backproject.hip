@@ -413,9 +413,13 @@ __global__ __launch_bounds__(256) void backproject_kernel(BpJob J)
 // source row and column, so the per-voxel row coordinate needs no clamp.
 //
 // Per voxel:
-//   fp32 / bf16 -- v FMA, fract, cvt, mad24, the taps (two ds_read2_b32 for
-//     fp32, one for 16-bit pairs), two lerps and two FMAs into the
-//     accumulator (10 VALU for fp32, +4 bit ops for bf16 pairs);
+//   fp32 -- v FMA, fract, cvt, mad24, the taps (two ds_read2_b32), two
+//     lerps and two FMAs into the accumulator (10 VALU);
+//   bf16 -- v FMA, fract, cvt, mad24, one ds_read2_b32 of the (u, u + 1)
+//     pairs of rows I and I + 1, three v_dot2_f32_bf16 with the column's
+//     bf16 (1 - wu, wu) weights (top row onto the accumulator, and the
+//     bottom-minus-top difference) and one FMA with the row weight
+//     (9 VALU; bf16 weights add about 2^-9 relative error per sample);
 //   the fp32 and fp16 loops are software-pipelined: the LDS reads of the
 //   next batch of voxels are issued before the current batch is combined;
 //   fp16 -- v FMA, fract, cvt, mad24, one ds_read2_b32 of the (u, u + 1)
@@ -477,6 +481,13 @@ __global__ __launch_bounds__(256) void backproject_kernel(BpJob J)
 // Measured on 512^3 x 720 and 1024^3 x 1200 (GUPS): fp32 batch 4 2767 /
 // 3254, batch 2 2792 / 3389, batch 8 2625 / 3071; packed fp16 without
 // pipelining 3343 / 3751, batch 2 3556 / 4146, batch 4 3487 / 4120.
+// bf16 pairs through v_dot2_f32_bf16 with bf16 column weights (1) or widened
+// to FP32 with FP32 weights (0).  Measured (GUPS, 512^3 / 1024^3 / 1440^3):
+// 3519 / 3954 / 4073 against 3070 / 3442 / 3692; rel RMSE on the 512^3
+// phantom scan 2.5e-4 against 1.6e-4.
+#ifndef CTR_BF16_DOT
+#define CTR_BF16_DOT 1
+#endif
 #ifndef CTR_F32_BATCH
 #define CTR_F32_BATCH 2
 #endif
@@ -632,6 +643,7 @@ __device__ inline LdsRun lds_run(const ColumnView &c, int u0, int v0, unsigned b
 
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 typedef __fp16 fp16x2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 
 template <int DT, int TY>
 __device__ inline void lds_run_accumulate(float (&acc)[TY], const LdsRun &R, const unsigned *lds,
@@ -711,6 +723,44 @@ __device__ inline void lds_run_accumulate(float (&acc)[TY], const LdsRun &R, con
                 const float left = fmaf(wv, b.x - a.x, a.x);
                 const float right = fmaf(wv, b.y - a.y, a.y);
                 acc[bt * kB + q] = fmaf(R.cR, right, fmaf(R.cL, left, acc[bt * kB + q]));
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    } else if constexpr (DT == CTR_BF16 && CTR_BF16_DOT) {
+        // bf16 pairs: each row's (u, u + 1) pair is weighted by a
+        // v_dot2_f32_bf16 with the column's bf16 (1 - wu, wu) w2 weights:
+        // acc += W.a in place, then d = W.b - W.a and acc += wv d.  No
+        // widening of the pairs is needed.
+        const bf16x2_t W = {(__bf16)R.cL, (__bf16)R.cR};
+        const bf16x2_t Wn = {-W.x, -W.y};
+        constexpr int kB = CTR_F32_BATCH;
+        unsigned ta[2][kB], tb[2][kB];
+        float tw[2][kB];
+        auto issue = [&](int set, int t0) {
+#pragma unroll
+            for (int q = 0; q < kB; ++q) {
+                const float x = fmaf(R.dv, (float)(t0 + q), R.f);
+                tw[set][q] = __builtin_amdgcn_fractf(x);
+                const unsigned *P = reinterpret_cast<const unsigned *>(
+                    reinterpret_cast<const char *>(lds) + __umul24((unsigned)x, lp4) + R.base);
+                ta[set][q] = P[0];
+                tb[set][q] = P[LdsConfig<DT>::kPitch];
+            }
+        };
+        issue(0, 0);
+#pragma unroll
+        for (int bt = 0; bt < TY / kB; ++bt) {
+            if (bt + 1 < TY / kB) issue((bt + 1) & 1, (bt + 1) * kB);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < kB; ++q) {
+                const bf16x2_t a = __builtin_bit_cast(bf16x2_t, ta[bt & 1][q]);
+                const bf16x2_t b = __builtin_bit_cast(bf16x2_t, tb[bt & 1][q]);
+                float &A = acc[bt * kB + q];
+                A = __builtin_amdgcn_fdot2_f32_bf16(a, W, A, false);
+                const float d = __builtin_amdgcn_fdot2_f32_bf16(
+                    a, Wn, __builtin_amdgcn_fdot2_f32_bf16(b, W, 0.0f, false), false);
+                A = fmaf(tw[bt & 1][q], d, A);
             }
             __builtin_amdgcn_sched_barrier(0);
         }

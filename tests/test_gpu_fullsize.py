@@ -198,6 +198,9 @@ def test_backproject_kernel_variants_agree(ctr, dev, monkeypatch):
         if dt == ctr.F16:
             assert rel_max(a, b) < 2e-3, dt
             assert rel_rmse(a, b) < 8e-4, dt
+        elif dt == ctr.BF16:
+            assert rel_max(a, b) < 8e-3, dt
+            assert rel_rmse(a, b) < 3e-3, dt
         else:
             assert rel_max(a, b) < 5e-6, dt
 
