@@ -268,12 +268,16 @@ __global__ __launch_bounds__(256) void view_max_from_rows_kernel(const float *ro
 }
 
 // FDK cosine weight SDD / sqrt(SDD^2 + u^2 + v^2) of detector column e for a
-// row with vv = SDD^2 + v^2: FP32 reciprocal square root (about 1e-7
-// relative error on a smooth weight; the row terms come from FP64).
-__device__ inline float fdk_weight_at(float sdd, float vv, float cu, float delta, int e)
+// row with vv = SDD^2 + v^2, in FP64: the hardware reciprocal square root
+// refined by one Newton step y (3 - x y^2) / 2 (about 2^-44 relative), a
+// handful of FP64 operations instead of a library sqrt and divide.
+__device__ inline float fdk_weight_at(double sdd, double vv, double cu, double delta, int e)
 {
-    const float u = ((float)e - cu) * delta;
-    return sdd * rsqrtf(fmaf(u, u, vv));
+    const double u = ((double)e - cu) * delta;
+    const double x = fma(u, u, vv);
+    double y = __builtin_amdgcn_rsq(x);
+    y *= fma(-0.5 * x * y, y, 1.5);
+    return (float)(sdd * y);
 }
 
 __device__ inline void row_coords(const FilterJob &J, int r, int &view, int &iv)
@@ -309,8 +313,7 @@ __global__ __launch_bounds__(kFftThreads) void filter_pairs_kernel(FilterJob J)
     }
     const float *a_row = J.in + (size_t)r0 * J.width;
     const float *b_row = J.in + (size_t)r1 * J.width;
-    const float sdd_f = (float)sqrt(J.sdd2), vv0_f = (float)vv0, vv1_f = (float)vv1;
-    const float cu_f = (float)cu, delta_f = (float)J.delta;
+    const double sdd = J.fdk ? sqrt(J.sdd2) : 0.0;
 
     load_twiddles<LOG2N>(T);
     for (int e = threadIdx.x; e < N; e += blockDim.x) {
@@ -319,8 +322,8 @@ __global__ __launch_bounds__(kFftThreads) void filter_pairs_kernel(FilterJob J)
             a = a_row[e];
             if (has1) b = b_row[e];
             if (J.fdk) {
-                a *= fdk_weight_at(sdd_f, vv0_f, cu_f, delta_f, e);
-                b *= fdk_weight_at(sdd_f, vv1_f, cu_f, delta_f, e);
+                a *= fdk_weight_at(sdd, vv0, cu, J.delta, e);
+                b *= fdk_weight_at(sdd, vv1, cu, J.delta, e);
             }
         }
         L[bitrev<LOG2N>(e)] = make_float2(a, b);
@@ -542,8 +545,6 @@ __global__ __launch_bounds__((1 << LOG2N) / 8) void filter_stockham_kernel(Filte
     }
     const float *a_row = J.in + (size_t)r0 * J.width;
     const float *b_row = J.in + (size_t)r1 * J.width;
-    const float sdd_f = (float)sdd, vv0_f = (float)vv0, vv1_f = (float)vv1;
-    const float cu_f = (float)cu, delta_f = (float)J.delta;
 
     // forward pass 0 (radix 8, span 1) straight from the detector rows
     float2 v[8];
@@ -555,8 +556,8 @@ __global__ __launch_bounds__((1 << LOG2N) / 8) void filter_stockham_kernel(Filte
             a = a_row[e];
             if (has1) b = b_row[e];
             if (J.fdk) {
-                a *= fdk_weight_at(sdd_f, vv0_f, cu_f, delta_f, e);
-                b *= fdk_weight_at(sdd_f, vv1_f, cu_f, delta_f, e);
+                a *= fdk_weight_at(sdd, vv0, cu, J.delta, e);
+                b *= fdk_weight_at(sdd, vv1, cu, J.delta, e);
             }
         }
         v[r] = make_float2(a, b);
