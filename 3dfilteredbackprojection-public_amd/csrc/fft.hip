@@ -1,12 +1,14 @@
 // fft.hip -- batched row FFTs, the Fourier-space ramp filter and the staged
 // projection layout, for gfx950.
 //
-// Rows are transformed out of LDS: one workgroup owns one complex row of
-// length n <= 8192, loads it in bit-reversed order, runs log2(n) radix-2
-// butterfly stages against an LDS twiddle table and writes it back.  The
-// fused filter packs two real detector rows into one complex FFT (the gain
-// is real and even, so the real and imaginary parts of the inverse carry the
-// two filtered rows independently).
+// The plain row FFTs (FFFT) run out of LDS: one workgroup owns one complex
+// row of length n <= 8192, loads it in bit-reversed order, runs log2(n)
+// radix-2 butterfly stages against an LDS twiddle table and writes it back.
+// The fused filter packs two real detector rows into one complex FFT (the
+// gain is real and even, so the real and imaginary parts of the inverse
+// carry the two filtered rows independently): for n = 512 .. 2048 one wave
+// owns the pair and exchanges data only through its own LDS buffer, for
+// larger n a workgroup runs radix-8 Stockham passes through LDS.
 #include <mutex>
 
 #include "ctr_internal.h"
@@ -546,21 +548,30 @@ __global__ __launch_bounds__((1 << LOG2N) / 8) void filter_stockham_kernel(Filte
     const float *a_row = J.in + (size_t)r0 * J.width;
     const float *b_row = J.in + (size_t)r1 * J.width;
 
-    // forward pass 0 (radix 8, span 1) straight from the detector rows
+    // forward pass 0 (radix 8, span 1) straight from the detector rows,
+    // through buffer resources sized to the rows (columns past the row and
+    // a missing second row read zero): the eight loads go out together
+    const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float *>(a_row), (short)0, J.width * 4, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float *>(has1 ? b_row : a_row), (short)0, has1 ? J.width * 4 : 0, 0x00020000);
     float2 v[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-        const int e = tid + r * T;
-        float a = 0.f, b = 0.f;
-        if (e < J.width) {
-            a = a_row[e];
-            if (has1) b = b_row[e];
-            if (J.fdk) {
-                a *= fdk_weight_at(sdd, vv0, cu, J.delta, e);
-                b *= fdk_weight_at(sdd, vv1, cu, J.delta, e);
+        const int off = (tid + r * T) * 4;
+        v[r] = make_float2(
+            __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(ra, off, 0, 0)),
+            __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rb, off, 0, 0)));
+    }
+    if (J.fdk) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int e = tid + r * T;
+            if (e < J.width) {
+                v[r].x *= fdk_weight_at(sdd, vv0, cu, J.delta, e);
+                v[r].y *= fdk_weight_at(sdd, vv1, cu, J.delta, e);
             }
         }
-        v[r] = make_float2(a, b);
     }
     SmallFFT<8, false>::run(v);
     stockham_write<N, 8>(v, L, 1, tid);
@@ -618,10 +629,411 @@ __global__ __launch_bounds__((1 << LOG2N) / 8) void filter_stockham_kernel(Filte
     if (J.row_max) note_row_max(J, ma, mb, r0, has1);
 }
 
+// ---------------------------------------------------------------------------
+// One-wave row filter (n = 512, 1024, 2048)
+// ---------------------------------------------------------------------------
+// A wave owns a pair of rows (one complex n-point FFT, n = 64 M) and keeps
+// it in registers: lane l holds x[l + 64 m], m < M.  The forward transform
+// is the four-step factorisation n = M x 64 with the 64-point part split
+// 8 x 8 (l = l1 + 8 l2, p = p2 + 8 p1, output k = q + M (p2 + 8 p1)):
+//   1. an M-point DFT over m in registers, times W_n^(l q);
+//   2. exchange through the wave's own LDS buffer, then 8-point DFTs over
+//      l2 (lanes l1 + 8 qb own q = 4 qb + j ...), times W_64^(l1 p2);
+//   3. exchange, then 8-point DFTs over l1 (lanes p2 + 8 qb).
+// The DFT matrix is symmetric, so the inverse runs the same steps in
+// reverse order with conjugate twiddles and ends in the input layout: no
+// reordering pass.  The exchanges are wave-private (no workgroup barrier;
+// LDS operations of one wave complete in order) and are done in two
+// halves over j to halve the buffer.  Workgroups are persistent: the
+// twiddle tables are built once in LDS and each wave loops over row pairs.
+
+// One-wave filter for n = 512 .. 2048 (1) or the Stockham kernel (0); j
+// groups per exchange round; minimum waves per SIMD the compiler must keep
+// registers for.  Measured on 512^3 x 720 (FFT 2048, fp32 staging, ms):
+// Stockham 2.81, one-wave 1.94 (2 waves per SIMD) / 1.86 (3, chosen);
+// 256^3 x 360 (FFT 1024) 0.337 / 0.214; two j groups per round 0.266.
+#ifndef CTR_WAVE_FILTER
+#define CTR_WAVE_FILTER 1
+#endif
+#ifndef CTR_WAVE_FILTER_JH
+#define CTR_WAVE_FILTER_JH 1
+#endif
+#ifndef CTR_WAVE_FILTER_MINW
+#define CTR_WAVE_FILTER_MINW 3
+#endif
+
+constexpr double cexpr_pi = 3.141592653589793238462643383279502884;
+
+// cos / sin of 2 pi e / n at compile time (Taylor series after folding the
+// angle into [-pi, pi]).
+constexpr double cexpr_cos_sin(int e, int n, bool want_sin)
+{
+    e %= n;
+    if (e < 0) e += n;
+    double x = 2.0 * cexpr_pi * (double)e / (double)n;
+    if (x > cexpr_pi) x -= 2.0 * cexpr_pi;
+    double term = want_sin ? x : 1.0, sum = term;
+    for (int k = 1; k < 30; ++k) {
+        const int a = want_sin ? 2 * k : 2 * k - 1, b = want_sin ? 2 * k + 1 : 2 * k;
+        term *= -x * x / ((double)a * (double)b);
+        sum += term;
+    }
+    return sum;
+}
+
+// x * W_n^e (forward) or x * conj(W_n^e), W_n = exp(-2 pi i / n), e known
+// at compile time.
+template <int N, int E, bool INV>
+__device__ inline float2 ctwiddle(float2 x)
+{
+    if constexpr (E % N == 0) {
+        return x;
+    } else {
+        constexpr float c = (float)cexpr_cos_sin(E, N, false);
+        constexpr float s = INV ? (float)cexpr_cos_sin(E, N, true)
+                                : -(float)cexpr_cos_sin(E, N, true);
+        return make_float2(x.x * c - x.y * s, x.x * s + x.y * c);
+    }
+}
+
+// M-point DFT (M = 8, 16, 32) of v in registers, natural order in and out:
+// M = 8 R, X[k8 + 8 kr] = sum_r W_R^(r kr) W_M^(r k8) sum_s W_8^(s k8) v[r + R s].
+template <int M, bool INV, int R>
+struct LaneDftStage {
+    template <int r, int k8>
+    __device__ static inline void tw(float2 (&a)[R][8])
+    {
+        if constexpr (k8 < 8) {
+            a[r][k8] = ctwiddle<M, r * k8, INV>(a[r][k8]);
+            tw<r, k8 + 1>(a);
+        }
+    }
+    template <int r>
+    __device__ static inline void tw_all(float2 (&a)[R][8])
+    {
+        if constexpr (r < R) {
+            tw<r, 1>(a);
+            tw_all<r + 1>(a);
+        }
+    }
+};
+
+template <int M, bool INV>
+__device__ inline void lane_dft(float2 (&v)[M])
+{
+    if constexpr (M == 8) {
+        SmallFFT<8, INV>::run(v);
+    } else {
+        constexpr int R = M / 8;
+        float2 a[R][8];
+        // scheduling barriers between the sub-DFTs keep the compiler from
+        // interleaving all of them (which needs twice the registers)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+#pragma unroll
+            for (int s = 0; s < 8; ++s) a[r][s] = v[r + R * s];
+            SmallFFT<8, INV>::run(a[r]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        LaneDftStage<M, INV, R>::template tw_all<1>(a);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int k8 = 0; k8 < 8; ++k8) {
+            float2 z[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) z[r] = a[r][k8];
+            SmallFFT<R, INV>::run(z);
+#pragma unroll
+            for (int kr = 0; kr < R; ++kr) v[k8 + 8 * kr] = z[kr];
+            if (k8 & 1) __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
+// Orders the LDS accesses of one wave across lanes (the hardware executes a
+// wave's LDS instructions in order; this keeps the compiler from moving
+// them across the exchange points).
+__device__ inline void wave_lds_fence()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+template <int LOG2N>
+struct WaveFft {
+    static constexpr int N = 1 << LOG2N, M = N / 64, J4 = M / 8;
+    static constexpr int JH = J4 >= 2 ? CTR_WAVE_FILTER_JH : 1, H = J4 / JH;
+    // row strides of the exchange buffer (complex elements), chosen so that
+    // the strided read of each exchange hits 16 distinct bank pairs
+    static constexpr int R1 = JH == 2 ? 68 : 72, R2 = 65;
+    static constexpr int kBuf = 8 * JH * (R1 > R2 ? R1 : R2);
+    static_assert(M == 8 || M == 16 || M == 32, "n = 512, 1024 or 2048");
+
+    // forward: input layout -> output layout (slot p1 J4 + j holds
+    // X[qb J4 + j + M (p2 + 8 p1)], lane = p2 + 8 qb)
+    __device__ static inline void forward(float2 (&v)[M], float2 *B, const float2 *TwM,
+                                          const float2 *Tw64, int lane)
+    {
+        lane_dft<M, false>(v);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = 1; q < M; ++q) {
+            v[q] = cmul(v[q], TwM[q * 64 + lane]);
+            if ((q & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+        }
+        const int lo = lane & 7, hi = lane >> 3;
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+            wave_lds_fence();
+#pragma unroll
+            for (int qb = 0; qb < 8; ++qb)
+#pragma unroll
+                for (int jj = 0; jj < JH; ++jj)
+                    B[(qb * JH + jj) * R1 + lane] = v[qb * J4 + h * JH + jj];
+            wave_lds_fence();
+            float2 y[JH][8];
+#pragma unroll
+            for (int jj = 0; jj < JH; ++jj)
+#pragma unroll
+                for (int l2 = 0; l2 < 8; ++l2) y[jj][l2] = B[(hi * JH + jj) * R1 + lo + 8 * l2];
+            wave_lds_fence();
+#pragma unroll
+            for (int jj = 0; jj < JH; ++jj) {
+                SmallFFT<8, false>::run(y[jj]);
+#pragma unroll
+                for (int p2 = 1; p2 < 8; ++p2) y[jj][p2] = cmul(y[jj][p2], Tw64[p2 * 8 + lo]);
+#pragma unroll
+                for (int p2 = 0; p2 < 8; ++p2) B[(jj * 8 + p2) * R2 + lane] = y[jj][p2];
+            }
+            wave_lds_fence();
+#pragma unroll
+            for (int jj = 0; jj < JH; ++jj) {
+                float2 z[8];
+#pragma unroll
+                for (int l1 = 0; l1 < 8; ++l1) z[l1] = B[(jj * 8 + lo) * R2 + l1 + 8 * hi];
+                SmallFFT<8, false>::run(z);
+#pragma unroll
+                for (int p1 = 0; p1 < 8; ++p1) v[p1 * J4 + h * JH + jj] = z[p1];
+            }
+        }
+    }
+
+    // inverse (unnormalised, conjugate twiddles): output layout -> input
+    // layout, the forward steps transposed
+    __device__ static inline void inverse(float2 (&v)[M], float2 *B, const float2 *TwM,
+                                          const float2 *Tw64, int lane)
+    {
+        const int lo = lane & 7, hi = lane >> 3;
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+            wave_lds_fence();
+#pragma unroll
+            for (int jj = 0; jj < JH; ++jj) {
+                float2 z[8];
+#pragma unroll
+                for (int p1 = 0; p1 < 8; ++p1) z[p1] = v[p1 * J4 + h * JH + jj];
+                SmallFFT<8, true>::run(z);
+#pragma unroll
+                for (int l1 = 1; l1 < 8; ++l1) {
+                    float2 w = Tw64[lo * 8 + l1];
+                    w.y = -w.y;
+                    z[l1] = cmul(z[l1], w);
+                }
+#pragma unroll
+                for (int l1 = 0; l1 < 8; ++l1) B[(jj * 8 + lo) * R2 + l1 + 8 * hi] = z[l1];
+            }
+            wave_lds_fence();
+            float2 y[JH][8];
+#pragma unroll
+            for (int jj = 0; jj < JH; ++jj)
+#pragma unroll
+                for (int p2 = 0; p2 < 8; ++p2) y[jj][p2] = B[(jj * 8 + p2) * R2 + lane];
+            wave_lds_fence();
+#pragma unroll
+            for (int jj = 0; jj < JH; ++jj) {
+                SmallFFT<8, true>::run(y[jj]);
+                const int q = hi * J4 + h * JH + jj;
+#pragma unroll
+                for (int l2 = 0; l2 < 8; ++l2) {
+                    if (q != 0 || l2 != 0) {
+                        float2 w = TwM[q * 64 + lo + 8 * l2];
+                        w.y = -w.y;
+                        y[jj][l2] = cmul(y[jj][l2], w);
+                    }
+                    B[(hi * JH + jj) * R1 + lo + 8 * l2] = y[jj][l2];
+                }
+            }
+            wave_lds_fence();
+#pragma unroll
+            for (int qb = 0; qb < 8; ++qb)
+#pragma unroll
+                for (int jj = 0; jj < JH; ++jj)
+                    v[qb * J4 + h * JH + jj] = B[(qb * JH + jj) * R1 + lane];
+        }
+        lane_dft<M, true>(v);
+    }
+};
+
+constexpr int kWaveFftWaves = 4;  // waves (row pairs in flight) per workgroup
+
+template <int LOG2N>
+__global__ __launch_bounds__(64 * kWaveFftWaves, CTR_WAVE_FILTER_MINW) void filter_wave_kernel(FilterJob J)
+{
+    using F = WaveFft<LOG2N>;
+    constexpr int N = F::N, M = F::M, J4 = F::J4;
+    __shared__ float2 TwM[M * 64];   // W_n^(l q), q < M, l < 64
+    __shared__ float2 Tw64[64];      // W_64^(l1 p2) at p2 * 8 + l1
+    __shared__ float2 Xch[kWaveFftWaves][F::kBuf];
+    const int tid = threadIdx.x, lane = tid & 63;
+    // wave-uniform in a scalar register: the row pair, its row coordinates
+    // and row addresses then stay scalar
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    constexpr int S = kMaxN / N;
+    auto twiddle = [](int e) {
+        const float2 w = g_twiddle[(e & (N / 2 - 1)) * S];
+        return e < N / 2 ? w : make_float2(-w.x, -w.y);
+    };
+    for (int i = tid; i < M * 64; i += 64 * kWaveFftWaves) TwM[i] = twiddle((i >> 6) * (i & 63));
+    if (tid < 64) Tw64[tid] = twiddle((N / 64) * (tid >> 3) * (tid & 7));
+    __syncthreads();
+
+    float2 *B = Xch[wave];
+    const int pairs = (J.rows + 1) / 2;
+    const double cu = 0.5 * J.width - 0.5;
+    const double sdd = J.fdk ? sqrt(J.sdd2) : 0.0;
+    const float inv_n = 1.0f / (float)N;
+#pragma unroll 1
+    for (int rp = blockIdx.x * kWaveFftWaves + wave; rp < pairs;
+         rp += gridDim.x * kWaveFftWaves) {
+        // the lane index made opaque per row pair: every lane-derived column,
+        // weight and gain address is recomputed in the loop rather than
+        // hoisted out of it, where the 32 per-column values would stay live
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
+        const int lo = ln & 7, hi = ln >> 3;
+        const int r0 = 2 * rp, r1 = r0 + 1;
+        const bool has1 = r1 < J.rows;
+        const int r1c = has1 ? r1 : r0;  // row coordinates of a valid row
+        const bool proj = J.layout == CTR_LAYOUT_PROJECTIONS;
+        const int iv0 = proj ? r0 % J.nv : r0 / J.n_views;
+        const int view0 = proj ? r0 / J.nv : r0 % J.n_views;
+        const int iv1 = proj ? r1c % J.nv : r1c / J.n_views;
+        const int view1 = proj ? r1c / J.nv : r1c % J.n_views;
+        double vv0 = 0.0, vv1 = 0.0;
+        if (J.fdk) {
+            const double v0 = (iv0 + 0.5 - 0.5 * J.nv) * J.delta;
+            const double v1 = (iv1 + 0.5 - 0.5 * J.nv) * J.delta;
+            vv0 = J.sdd2 + v0 * v0;
+            vv1 = J.sdd2 + v1 * v1;
+        }
+        // both rows through buffer resources sized to the row: columns past
+        // the row (and the missing second row of an odd count) read zero,
+        // so all loads are issued together without branches
+        const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float *>(J.in + (size_t)r0 * J.width), (short)0, J.width * 4, 0x00020000);
+        const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float *>(J.in + (size_t)(has1 ? r1 : r0) * J.width), (short)0,
+            has1 ? J.width * 4 : 0, 0x00020000);
+        float2 v[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const int off = (ln + 64 * m) * 4;
+            v[m] = make_float2(__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                                             ra, off, 0, 0)),
+                               __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                                             rb, off, 0, 0)));
+        }
+        if (J.fdk) {
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                if (64 * m < J.width) {  // wave-uniform: skips all-zero groups
+                    const int e = ln + 64 * m;
+                    v[m].x *= fdk_weight_at(sdd, vv0, cu, J.delta, e);
+                    v[m].y *= fdk_weight_at(sdd, vv1, cu, J.delta, e);
+                }
+            }
+        }
+        F::forward(v, B, TwM, Tw64, ln);
+        const float *gain = g_gain[J.filter] + N;
+#pragma unroll
+        for (int j = 0; j < J4; ++j)
+#pragma unroll
+            for (int p1 = 0; p1 < 8; ++p1) {
+                const float g = gain[hi * J4 + j + M * (lo + 8 * p1)];
+                float2 &x = v[p1 * J4 + j];
+                x = make_float2(x.x * g, x.y * g);
+                if ((p1 & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+            }
+        F::inverse(v, B, TwM, Tw64, ln);
+
+        float *oa, *ob;
+        if (J.staged == nullptr) {
+            oa = J.out + (size_t)r0 * J.width;
+            ob = J.out + (size_t)r1 * J.width;
+        } else {
+            const size_t vstride = (size_t)J.pitch_u * J.pitch_v;
+            oa = J.staged + (size_t)view0 * vstride + (size_t)(iv0 + 1) * J.pitch_u + 1;
+            ob = J.staged + (size_t)view1 * vstride + (size_t)(iv1 + 1) * J.pitch_u + 1;
+        }
+        float ma = 0.0f, mb = 0.0f;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const int e = ln + 64 * m;
+            if (e < J.width) {
+                const float x = v[m].x * inv_n, y = v[m].y * inv_n;
+                oa[e] = x;
+                if (has1) ob[e] = y;
+                ma = fmaxf(ma, fabsf(x));
+                mb = fmaxf(mb, fabsf(y));
+            }
+        }
+        if (J.row_max) {
+            for (int off = 32; off > 0; off >>= 1) {
+                ma = fmaxf(ma, __shfl_xor(ma, off));
+                mb = fmaxf(mb, __shfl_xor(mb, off));
+            }
+            if (lane == 0) {
+                J.row_max[r0] = ma;
+                if (has1) J.row_max[r1] = mb;
+            }
+        }
+    }
+}
+
+// Workgroups of the one-wave filter: as many as are resident at once on
+// the device (persistent), or one per four row pairs for small jobs.
+template <int LOG2N>
+static void launch_filter_wave(hipStream_t s, const FilterJob &J, int pairs)
+{
+    static int resident = 0;  // per process; the device is the same MI355X model throughout
+    if (resident == 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) ==
+                hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, filter_wave_kernel<LOG2N>, 64 * kWaveFftWaves, 0) == hipSuccess &&
+            cus > 0 && per_cu > 0)
+            resident = cus * per_cu;
+        else
+            resident = 1024;
+    }
+    const int want = (pairs + kWaveFftWaves - 1) / kWaveFftWaves;
+    filter_wave_kernel<LOG2N><<<want < resident ? want : resident, 64 * kWaveFftWaves, 0, s>>>(J);
+}
+
 static int launch_filter_pairs(hipStream_t s, const FilterJob &J, int log2n)
 {
     int blocks = (J.rows + 1) / 2;
     if (blocks == 0) return CTR_OK;
+    if (CTR_WAVE_FILTER && log2n >= 9 && log2n <= 11) {
+        if (log2n == 9) launch_filter_wave<9>(s, J, blocks);
+        if (log2n == 10) launch_filter_wave<10>(s, J, blocks);
+        if (log2n == 11) launch_filter_wave<11>(s, J, blocks);
+        CTR_CHECK_LAUNCH();
+        return CTR_OK;
+    }
 #define CTR_FILT_CASE(L)                                                          \
     case L:                                                                       \
         filter_pairs_kernel<L><<<blocks, kFftThreads, 0, s>>>(J);                 \

@@ -42,7 +42,8 @@ def test_fft_rejects_non_power_of_two(ctr, dev):
 
 @pytest.mark.parametrize("width,npad,ftype", [
     (367, 0, 0), (367, 1024, 2), (768, 2048, 0), (768, 2048, 1), (768, 2048, 2),
-    (100, 128, 0), (1, 1, 0), (5, 8, 2), (1536, 4096, 0)])
+    (100, 128, 0), (1, 1, 0), (5, 8, 2), (1536, 4096, 0),
+    (512, 512, 0), (65, 1024, 1), (1000, 2048, 2), (2048, 2048, 0)])
 def test_filter_rows_vs_oracle(ctr, orc, dev, width, npad, ftype):
     rng = np.random.default_rng(width + npad)
     rows = 7
@@ -68,14 +69,14 @@ def _cone(ctr, **kw):
     return ctr.Geometry(**base)
 
 
-@pytest.mark.parametrize("layout", [0, 1])
-def test_filter_projections_fdk_vs_oracle(ctr, orc, dev, layout):
+@pytest.mark.parametrize("layout,nfft", [(0, 256), (1, 256), (0, 1024), (1, 1024)])
+def test_filter_projections_fdk_vs_oracle(ctr, orc, dev, layout, nfft):
     import torch
     g = _cone(ctr)
     rng = np.random.default_rng(11)
     proj = rng.standard_normal((g.n_views, g.nv, g.nu)).astype(np.float32)
     src = proj if layout == 0 else np.ascontiguousarray(proj.transpose(1, 0, 2))
-    staged, _ = ctr.filter_projections(_t(dev, src), g, 0, 256, ctr.F32, fdk=True,
+    staged, _ = ctr.filter_projections(_t(dev, src), g, 0, nfft, ctr.F32, fdk=True,
                                        layout=layout)
     pu, pv, _ = ctr.staged_layout(g.nu, g.nv, g.n_views)
     st = staged.view(torch.float32).cpu().numpy().reshape(g.n_views, pv, pu)
@@ -83,7 +84,7 @@ def test_filter_projections_fdk_vs_oracle(ctr, orc, dev, layout):
     assert np.all(st[:, :, 0] == 0) and np.all(st[:, :, g.nu + 1:] == 0)
     got = st[:, 1:g.nv + 1, 1:g.nu + 1]
     w = orc.fdk_weight(proj, g.source_distance, g.detector_distance, g.detector_width)
-    ref = orc.filter_rows(w.astype(np.float32), 0, 256)
+    ref = orc.filter_rows(w.astype(np.float32), 0, nfft)
     assert rel_max(got, ref) < 1e-5
 
 
