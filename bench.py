@@ -193,8 +193,9 @@ def main():
                     help="serving slots per class: small,medium,large")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(z-slab sharding is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path "
+              "(the reconstruction would shard into z-slabs with no exchange; "
+              "a slab is the same kernel with fewer y-runs)", file=sys.stderr)
         sys.exit(2)
     if args.mode == "serving":
         serving_main(args)
