@@ -638,14 +638,17 @@ __global__ __launch_bounds__((1 << LOG2N) / 8) void filter_stockham_kernel(Filte
 // 8 x 8 (l = l1 + 8 l2, p = p2 + 8 p1, output k = q + M (p2 + 8 p1)):
 //   1. an M-point DFT over m in registers, times W_n^(l q);
 //   2. exchange through the wave's own LDS buffer, then 8-point DFTs over
-//      l2 (lanes l1 + 8 qb own q = 4 qb + j ...), times W_64^(l1 p2);
+//      l2 (lane l1 + 8 qb owns q = (M/8) qb + j, j < M/8), times
+//      W_64^(l1 p2);
 //   3. exchange, then 8-point DFTs over l1 (lanes p2 + 8 qb).
 // The DFT matrix is symmetric, so the inverse runs the same steps in
 // reverse order with conjugate twiddles and ends in the input layout: no
 // reordering pass.  The exchanges are wave-private (no workgroup barrier;
-// LDS operations of one wave complete in order) and are done in two
-// halves over j to halve the buffer.  Workgroups are persistent: the
-// twiddle tables are built once in LDS and each wave loops over row pairs.
+// LDS operations of one wave complete in order) and run one j group per
+// round, which keeps the buffer at 8 rows of 72 complex values.  The rows
+// are read through buffer resources, so all loads of a row pair are issued
+// together.  Workgroups are persistent: the twiddle tables are built once
+// in LDS and each wave loops over row pairs.
 
 // One-wave filter for n = 512 .. 2048 (1) or the Stockham kernel (0); j
 // groups per exchange round; minimum waves per SIMD the compiler must keep
