@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(HERE, "libctrecon.so")
 
 CTR_OK = 0
 F32, F16, BF16 = 0, 1, 2
+INTERP_FAST, INTERP_FP32 = 0, 1
 RAM_LAK, SHEPP_LOGAN, COSINE = 0, 1, 2
 CONE, PARALLEL = 0, 1
 PROJECTIONS, SINOGRAMS = 0, 1
@@ -86,6 +87,8 @@ SIGNATURES = {
     "ctr_min_max": [_v, _v, _v, _sz, _v],
     "ctr_normalize_range": [_v, _v, _sz, _v],
     "ctr_slices_unorm16": [_v, _v, _i, _i, _i, _i, _i, _v],
+    "ctr_set_interpolation": [_i],
+    "ctr_get_interpolation": [],
 }
 _RESTYPE = {"ctr_last_error": ctypes.c_char_p}
 
@@ -333,6 +336,19 @@ def backproject(staged, geom, dtype=F32, view_scale=None, scale=1.0, volume=None
         geom.X, geom.Y, geom.Z, float(scale), 1 if accumulate else 0,
         1 if clip else 0))
     return volume
+
+
+def set_interpolation(mode):
+    """Process-wide interpolation arithmetic of the 16-bit staged types:
+    INTERP_FAST or INTERP_FP32 (ctr_set_interpolation); returns the previous
+    mode."""
+    rc = lib().ctr_set_interpolation(int(mode))
+    check(rc if rc < 0 else CTR_OK)
+    return rc
+
+
+def get_interpolation():
+    return lib().ctr_get_interpolation()
 
 
 def backproject_packed(items, dtype=F32, ws=None, stream=None):

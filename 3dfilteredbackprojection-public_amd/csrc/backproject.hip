@@ -19,6 +19,7 @@
 //
 // A 256-thread block is 4 wave64s, each covering an 8 x 8 patch of columns,
 // so the 64 lanes of a wave sample a compact detector footprint.
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 
@@ -481,13 +482,6 @@ __global__ __launch_bounds__(256) void backproject_kernel(BpJob J)
 // Measured on 512^3 x 720 and 1024^3 x 1200 (GUPS): fp32 batch 4 2767 /
 // 3254, batch 2 2792 / 3389, batch 8 2625 / 3071; packed fp16 without
 // pipelining 3343 / 3751, batch 2 3556 / 4146, batch 4 3487 / 4120.
-// bf16 pairs through v_dot2_f32_bf16 with bf16 column weights (1) or widened
-// to FP32 with FP32 weights (0).  Measured (GUPS, 512^3 / 1024^3 / 1440^3):
-// 3519 / 3954 / 4073 against 3070 / 3442 / 3692; rel RMSE on the 512^3
-// phantom scan 2.5e-4 against 1.6e-4.
-#ifndef CTR_BF16_DOT
-#define CTR_BF16_DOT 1
-#endif
 #ifndef CTR_F32_BATCH
 #define CTR_F32_BATCH 2
 #endif
@@ -495,7 +489,10 @@ __global__ __launch_bounds__(256) void backproject_kernel(BpJob J)
 #define CTR_F16_BATCH 2
 #endif
 constexpr int kRectChunk = 128; // views per refill of the rectangle table
-constexpr int kF16Packed = 16;  // tile variant: fp16 pairs with packed f16 arithmetic
+// Tile variants with 16-bit interpolation arithmetic (the plain format tags
+// CTR_F16 / CTR_BF16 widen the pairs and interpolate in FP32):
+constexpr int kF16Packed = 16;  // fp16 pairs, packed f16 lerp, v_dot2_f32_f16 column weights
+constexpr int kBF16Dot = 17;    // bf16 pairs, v_dot2_f32_bf16 with bf16 column weights
 
 template <int DT>
 struct LdsConfig {
@@ -726,7 +723,7 @@ __device__ inline void lds_run_accumulate(float (&acc)[TY], const LdsRun &R, con
             }
             __builtin_amdgcn_sched_barrier(0);
         }
-    } else if constexpr (DT == CTR_BF16 && CTR_BF16_DOT) {
+    } else if constexpr (DT == kBF16Dot) {
         // bf16 pairs: each row's (u, u + 1) pair is weighted by a
         // v_dot2_f32_bf16 with the column's bf16 (1 - wu, wu) w2 weights:
         // acc += W.a in place, then d = W.b - W.a and acc += wv d.  No
@@ -765,16 +762,57 @@ __device__ inline void lds_run_accumulate(float (&acc)[TY], const LdsRun &R, con
             __builtin_amdgcn_sched_barrier(0);
         }
     } else {
+        // fp16 / bf16 pairs widened to FP32 and interpolated like the fp32
+        // branch; the pairs are read as raw dwords and widened after the
+        // scheduling barrier, so the reads of batch b + 1 stay in flight
+        static_assert(DT == CTR_F16 || DT == CTR_BF16, "tile variant");
+        constexpr int kB = CTR_F32_BATCH;
+        // 1.0 held in an SGPR: b * one - a on widened f16 halves then stays
+        // one mixed-precision FMA (a literal 1.0 is folded into a
+        // subtraction of two converted values)
+        float one = 1.0f;
+        asm("" : "+s"(one));
+        unsigned ta[2][kB], tb[2][kB];
+        float tw[2][kB];
+        auto issue = [&](int set, int t0) {
 #pragma unroll
-        for (int t = 0; t < TY; ++t) {
-            const float x = fmaf(R.dv, (float)t, R.f);
-            const float wv = __builtin_amdgcn_fractf(x);
-            const unsigned addr = __umul24((unsigned)x, lp4) + R.base;
-            float2 a, b;
-            lds_taps<DT>(lds, addr, a, b);
-            const float left = fmaf(wv, b.x - a.x, a.x);
-            const float right = fmaf(wv, b.y - a.y, a.y);
-            acc[t] = fmaf(R.cR, right, fmaf(R.cL, left, acc[t]));
+            for (int q = 0; q < kB; ++q) {
+                const float x = fmaf(R.dv, (float)(t0 + q), R.f);
+                tw[set][q] = __builtin_amdgcn_fractf(x);
+                const unsigned *P = reinterpret_cast<const unsigned *>(
+                    reinterpret_cast<const char *>(lds) + __umul24((unsigned)x, lp4) + R.base);
+                ta[set][q] = P[0];
+                tb[set][q] = P[LdsConfig<DT>::kPitch];
+            }
+        };
+        issue(0, 0);
+#pragma unroll
+        for (int bt = 0; bt < TY / kB; ++bt) {
+            if (bt + 1 < TY / kB) issue((bt + 1) & 1, (bt + 1) * kB);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < kB; ++q) {
+                const unsigned ra = ta[bt & 1][q], rb = tb[bt & 1][q];
+                const float wv = tw[bt & 1][q];
+                float &A = acc[bt * kB + q];
+                if constexpr (DT == CTR_F16) {
+                    // b - a and the lerp straight from the f16 halves: the
+                    // mixed-precision FMA widens them exactly, so this is
+                    // the FP32 arithmetic of the fp32 branch without the
+                    // four conversions
+                    const half2_t a = __builtin_bit_cast(half2_t, ra);
+                    const half2_t b = __builtin_bit_cast(half2_t, rb);
+                    const float left = fmaf(wv, fmaf((float)b.x, one, -(float)a.x), (float)a.x);
+                    const float right = fmaf(wv, fmaf((float)b.y, one, -(float)a.y), (float)a.y);
+                    A = fmaf(R.cR, right, fmaf(R.cL, left, A));
+                } else {
+                    const float2 a = bf16_pair(ra), b = bf16_pair(rb);
+                    const float left = fmaf(wv, b.x - a.x, a.x);
+                    const float right = fmaf(wv, b.y - a.y, a.y);
+                    A = fmaf(R.cR, right, fmaf(R.cL, left, A));
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
 }
@@ -793,7 +831,8 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
 {
     using L = LdsLayout<DT>;
     constexpr int HMAX = L::kRows, CMAX = L::kCols;
-    constexpr int kTapDT = DT == kF16Packed ? CTR_F16 : DT;  // staged texel format
+    // staged texel format of the variant
+    constexpr int kTapDT = DT == kF16Packed ? CTR_F16 : DT == kBF16Dot ? CTR_BF16 : DT;
     int4 *rects = reinterpret_cast<int4 *>(lds);
     unsigned *bufs = lds + L::kRectBytes / 4;
     const int pitch_u = J.pitch_u, pitch_v = J.pitch_v, N = J.n_views;
@@ -1046,14 +1085,42 @@ static void launch_lds_packed(hipStream_t s, BpBatch B, unsigned blocks)
     backproject_lds_packed_kernel<DT, kTY><<<blocks, 256, 0, s>>>(B);
 }
 
+// Interpolation arithmetic of the 16-bit formats, process-wide: CTR_INTERP_FP32
+// or CTR_INTERP_FAST (the kF16Packed / kBF16Dot variants).  The initial mode
+// is CTR_INTERP_FAST when the environment sets CTR_INTERPOLATION=fast.
+static std::atomic<int> g_interp{-1};
+
+static int interp_mode()
+{
+    int m = g_interp.load(std::memory_order_relaxed);
+    if (m < 0) {
+        const char *e = getenv("CTR_INTERPOLATION");
+        const int init = e && !strcmp(e, "fast") ? CTR_INTERP_FAST : CTR_INTERP_FP32;
+        g_interp.compare_exchange_strong(m, init);
+        m = g_interp.load(std::memory_order_relaxed);
+    }
+    return m;
+}
+
+// LDS tile variant of a staged format under the interpolation mode.
+static int lds_variant(int staged_dtype, bool f16_weights_ok)
+{
+    if (interp_mode() == CTR_INTERP_FP32) return staged_dtype;
+    if (staged_dtype == CTR_F16) return f16_weights_ok ? kF16Packed : CTR_F16;
+    if (staged_dtype == CTR_BF16) return kBF16Dot;
+    return staged_dtype;
+}
+
 template <int DT>
-static int launch_bp(hipStream_t s, const BpJob &J, bool packed_f16)
+static int launch_bp(hipStream_t s, const BpJob &J, int variant)
 {
     if (use_lds_kernel(J.Y)) {
         const long long blocks = (long long)J.gx * J.gy * ((J.Y + kTY - 1) / kTY);
         CTR_REQUIRE(blocks < (1LL << 31), CTR_ERR_INVALID_ARGUMENT, "volume too large");
-        if (DT == CTR_F16 && packed_f16)
+        if (variant == kF16Packed)
             launch_lds<kF16Packed>(s, J, (unsigned)blocks);
+        else if (variant == kBF16Dot)
+            launch_lds<kBF16Dot>(s, J, (unsigned)blocks);
         else
             launch_lds<DT>(s, J, (unsigned)blocks);
         CTR_CHECK_LAUNCH();
@@ -1123,13 +1190,26 @@ int ctr_backproject(void *stream, ctr_workspace *ws, int geometry, const void *s
         if ((rc = workspace_get(w, SLOT_FOOTPRINT, footprint_bytes(X, Z, n_views), &f))) return rc;
         if ((rc = build_footprints(s, J, (float4 *)f))) return rc;
     }
-    const bool packed = f16_weights_fit(J, geometry, source_distance, volume_width);
+    const int variant =
+        lds_variant(staged_dtype, f16_weights_fit(J, geometry, source_distance, volume_width));
     switch (staged_dtype) {
-    case CTR_F32: return launch_bp<CTR_F32>(s, J, packed);
-    case CTR_F16: return launch_bp<CTR_F16>(s, J, packed);
-    default: return launch_bp<CTR_BF16>(s, J, packed);
+    case CTR_F32: return launch_bp<CTR_F32>(s, J, variant);
+    case CTR_F16: return launch_bp<CTR_F16>(s, J, variant);
+    default: return launch_bp<CTR_BF16>(s, J, variant);
     }
 }
+
+int ctr_set_interpolation(int mode)
+{
+    clear_error();
+    CTR_REQUIRE(mode == CTR_INTERP_FAST || mode == CTR_INTERP_FP32, CTR_ERR_INVALID_ARGUMENT,
+                "ctr_set_interpolation: unknown mode %d", mode);
+    const int prev = interp_mode();
+    g_interp.store(mode);
+    return prev;
+}
+
+int ctr_get_interpolation(void) { return interp_mode(); }
 
 int ctr_backproject_packed(void *stream, ctr_workspace *ws, int staged_dtype,
                            const ctr_backproject_job *jobs, int count)
@@ -1191,15 +1271,12 @@ int ctr_backproject_packed(void *stream, ctr_workspace *ws, int staged_dtype,
         fp += footprint_bytes(j.X, j.Z, j.n_views);
     }
     CTR_REQUIRE(blocks < (1LL << 31), CTR_ERR_INVALID_ARGUMENT, "packed launch too large");
-    switch (staged_dtype) {
+    switch (lds_variant(staged_dtype, packed)) {
     case CTR_F32: launch_lds_packed<CTR_F32>(s, B, (unsigned)blocks); break;
-    case CTR_F16:
-        if (packed)
-            launch_lds_packed<kF16Packed>(s, B, (unsigned)blocks);
-        else
-            launch_lds_packed<CTR_F16>(s, B, (unsigned)blocks);
-        break;
-    default: launch_lds_packed<CTR_BF16>(s, B, (unsigned)blocks); break;
+    case CTR_F16: launch_lds_packed<CTR_F16>(s, B, (unsigned)blocks); break;
+    case CTR_BF16: launch_lds_packed<CTR_BF16>(s, B, (unsigned)blocks); break;
+    case kF16Packed: launch_lds_packed<kF16Packed>(s, B, (unsigned)blocks); break;
+    default: launch_lds_packed<kBF16Dot>(s, B, (unsigned)blocks); break;
     }
     CTR_CHECK_LAUNCH();
     return CTR_OK;

@@ -6,7 +6,9 @@ cone backprojection of all 720 views.  Inputs are synthetic: 30 seeded random
 ellipsoids rendered at 0.5 mm, projected by the ray-driven forward projector,
 Poisson noise at I0 = 1e5 and log.  The headline value is the FP16-stored
 form (filtered projections in FP16 with a per-view scale, FP32 interpolation
-and accumulation); the FP32-stored form is timed beside it.
+and accumulation); the FP32-stored and BF16-stored forms are timed beside it,
+and the 16-bit forms once more with the 16-bit interpolation arithmetic
+(ctr_set_interpolation CTR_INTERP_FAST) as a secondary figure.
 
 Prints ONE JSON line.  Roofline for the dominant kernel (backprojection):
 algorithmic bytes per voxel-view update = 4 taps x element size + 8/N bytes
@@ -226,6 +228,7 @@ def main():
     log(f"workload {WORKLOAD}: {g.X}x{g.Y}x{g.Z} from {g.n_views} views of "
         f"{g.nu}x{g.nv}, FFT {FFT_LENGTH}")
     proj = make_inputs(ctr, g, dev)
+    ctr.set_interpolation(ctr.INTERP_FP32)
 
     log("timing FP32-stored form")
     f32_ms, f32_bp, vol32 = time_fdk(ctr, g, proj, ctr.F32, args.steps, args.warmup, ws, scale)
@@ -239,6 +242,20 @@ def main():
     log("timing BF16-stored form")
     b16_ms, b16_bp, volb = time_fdk(ctr, g, proj, ctr.BF16, max(3, args.steps // 2), 1, ws, scale)
     rel_rmse_bf = float((volb.double() - ref32.double()).norm() / ref32.double().norm())
+    del volb
+    log("timing FP16 / BF16 with the 16-bit interpolation arithmetic")
+    ctr.set_interpolation(ctr.INTERP_FAST)
+    fast = {}
+    for name, dt in (("fp16", ctr.F16), ("bf16", ctr.BF16)):
+        t_ms, t_bp, v = time_fdk(ctr, g, proj, dt, max(3, args.steps // 2), 1, ws, scale)
+        fast[name] = {
+            "gups": round(g.updates / (statistics.mean(t_ms) * 1e-3) / 1e9, 3),
+            "ms_per_step": round(statistics.mean(t_ms), 3),
+            "backproject_ms": round(statistics.mean(t_bp), 3),
+            "rel_rmse_vs_fp32": float((v.double() - ref32.double()).norm()
+                                      / ref32.double().norm())}
+        del v
+    ctr.set_interpolation(ctr.INTERP_FP32)
 
     torch.cuda.synchronize()
     ms = statistics.mean(f16_ms)
@@ -260,7 +277,8 @@ def main():
         log(f"  {cpu['value']:.3f} GUPS")
 
     out = {
-        "metric": "FDK GUPS 512^3 x 720 views (FP16-stored filtered projections, FP32 accumulation)",
+        "metric": "FDK GUPS 512^3 x 720 views (FP16-stored filtered projections, FP32 "
+                  "interpolation and accumulation)",
         "value": round(gups, 3),
         "unit": "GUPS",
         "n_gpus": 1,
@@ -285,11 +303,13 @@ def main():
         "rel_rmse_fp16_vs_fp32": rel_rmse_16,
         "bf16_store": {"gups": round(g.updates / (statistics.mean(b16_ms) * 1e-3) / 1e9, 3),
                        "rel_rmse_vs_fp32": rel_rmse_bf},
+        "interp_16bit_arithmetic": fast,
         "backproject_gups_fp16": round(g.updates / (bp_ms * 1e-3) / 1e9, 3),
         "roofline": {"bound": "hbm", "achieved": round(ach16, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(ach16 / HBM_PEAK_GBPS, 4),
                      "traffic": traffic,
-                     "kernel": "backproject_lds_kernel<fp16 packed, 32> (LDS 48 columns, pitch 51, 72 rows)",
+                     "kernel": "backproject_lds_kernel<fp16, 32> (FP32 interpolation; LDS 48 "
+                               "columns, pitch 51, 72 rows)",
                      "avg_ms": round(bp_ms, 3)},
         "cpu_baseline": cpu,
     }

@@ -50,6 +50,10 @@ extern "C" {
 #define CTR_F16 1
 #define CTR_BF16 2
 
+/* interpolation arithmetic of the 16-bit staged types (ctr_set_interpolation) */
+#define CTR_INTERP_FAST 0
+#define CTR_INTERP_FP32 1
+
 /* FBP2D::FilterType (Source/FilteredBackProjection/FBP2D.h:10-14) */
 #define CTR_RAM_LAK 0
 #define CTR_SHEPP_LOGAN 1
@@ -202,12 +206,13 @@ int ctr_stage_projections(void *stream, ctr_workspace *ws, const float *proj,
  * max(v, 0) (FBP3D.cpp:444-458, run after every reference 3D
  * backprojection).  view_scale may be NULL for CTR_F32.  Coordinates are set
  * up in FP64 per column and view and stepped in FP32 along y; accumulation
- * is FP32.  Interpolation is FP32 for CTR_F32, and for every format on
- * volumes under 32 voxels along y.  Otherwise the 16-bit formats use their
- * own arithmetic for the interpolation weights: CTR_F16 pairs are
- * interpolated in packed f16 (per-sample relative error ~2^-11; about 4e-4
- * rel RMSE on white-noise projections, 4e-5 on the 512^3 phantom scan)
- * wherever the distance weight (SOD / (SOD - r))^2 of the volume's half
+ * is FP32.  Interpolation is FP32 (16-bit pairs widened exactly) under
+ * CTR_INTERP_FP32, the default mode of ctr_set_interpolation, and for every
+ * format on volumes under 32 voxels along y.  Under CTR_INTERP_FAST the
+ * 16-bit formats use their own arithmetic for the interpolation weights:
+ * CTR_F16 pairs are interpolated in packed f16 (per-sample relative error
+ * ~2^-11; about 4e-4 rel RMSE on white-noise projections, 4e-5 on the 512^3
+ * phantom scan, against 2e-5 with FP32 interpolation) wherever the distance weight (SOD / (SOD - r))^2 of the volume's half
  * diagonal r stays below 1024, in FP32 otherwise; CTR_BF16 pairs are
  * weighted with bf16 column weights by v_dot2_f32_bf16 with FP32 products
  * and sums (about 1.7e-3 rel RMSE on white-noise projections, 2.5e-4 on the
@@ -242,6 +247,19 @@ typedef struct ctr_backproject_job {
  * counterpart (the reference reconstructs one volume at a time). */
 int ctr_backproject_packed(void *stream, ctr_workspace *ws, int staged_dtype,
                            const ctr_backproject_job *jobs, int count);
+
+/* Interpolation arithmetic of ctr_backproject and ctr_backproject_packed for
+ * CTR_F16 / CTR_BF16 staged projections, for the whole process and every
+ * later call: CTR_INTERP_FP32 (pairs widened to FP32, interpolated and
+ * weighted in FP32) or CTR_INTERP_FAST (the 16-bit arithmetic described at
+ * ctr_backproject: 10-15% faster, larger error).  The initial mode is
+ * CTR_INTERP_FAST when the environment variable CTR_INTERPOLATION is "fast"
+ * at the first call, else CTR_INTERP_FP32.  ctr_set_interpolation returns
+ * the previous mode, or
+ * CTR_ERR_INVALID_ARGUMENT for an unknown one.  No reference counterpart
+ * (the reference interpolates FP32 texels in the sampler). */
+int ctr_set_interpolation(int mode);
+int ctr_get_interpolation(void);
 
 /* ---- forward projection ------------------------------------------------------
  * Ray-driven projector, one ray per detector pixel centre, replacing

@@ -78,3 +78,33 @@ def test_product_path_fails_loudly_without_library(ctr, monkeypatch):
     monkeypatch.setattr(ctr, "LIB_PATH", os.path.join(ROOT, "no_such_libctrecon.so"))
     with pytest.raises(RuntimeError, match="no CPU fallback"):
         ctr.lib()
+
+
+def test_interpolation_mode_switch(ctr):
+    """ctr_set_interpolation: returns the previous mode, rejects unknown ones
+    and leaves the mode unchanged then; no device is touched."""
+    lib = ctr.lib()
+    start = lib.ctr_get_interpolation()
+    assert start in (ctr.INTERP_FAST, ctr.INTERP_FP32)
+    try:
+        assert ctr.set_interpolation(ctr.INTERP_FP32) == start
+        assert ctr.get_interpolation() == ctr.INTERP_FP32
+        assert lib.ctr_set_interpolation(7) == -1
+        assert "unknown mode" in ctr.last_error()
+        assert ctr.get_interpolation() == ctr.INTERP_FP32
+        with pytest.raises(ctr.CtrError):
+            ctr.set_interpolation(-1)
+        assert ctr.set_interpolation(ctr.INTERP_FAST) == ctr.INTERP_FP32
+        assert ctr.get_interpolation() == ctr.INTERP_FAST
+    finally:
+        ctr.set_interpolation(start)
+
+
+@pytest.mark.parametrize("value,mode", [("fp32", 1), ("fast", 0), ("", 1)])
+def test_interpolation_mode_from_environment(ctr, value, mode):
+    code = ("import ctypes, sys; L = ctypes.CDLL(sys.argv[1]); "
+            "print(L.ctr_get_interpolation())")
+    env = dict(os.environ, CTR_INTERPOLATION=value)
+    out = subprocess.run(["python", "-c", code, ctr.LIB_PATH], env=env,
+                         capture_output=True, text=True, check=True).stdout
+    assert int(out.strip()) == mode
