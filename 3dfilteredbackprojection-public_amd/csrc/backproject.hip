@@ -488,6 +488,16 @@ __global__ __launch_bounds__(256) void backproject_kernel(BpJob J)
 #ifndef CTR_F16_BATCH
 #define CTR_F16_BATCH 2
 #endif
+// The same for the FP32-interpolation branch of the 16-bit pairs.  Measured
+// (GUPS, 512^3 / 1024^3 / 1440^3): fp16 batch 1 3420 / 3836 / 3965, batch 2
+// 3379 / 3682 / 3879, batch 4 3381 / 3727 at 512^3 / 1024^3; bf16 batch 2
+// 3203 / 3578 / 3791, batch 1 3172 / 3568 / 3790.
+#ifndef CTR_WF16_BATCH
+#define CTR_WF16_BATCH 1  // fp16 pairs, FP32 interpolation
+#endif
+#ifndef CTR_WBF16_BATCH
+#define CTR_WBF16_BATCH 2  // bf16 pairs, FP32 interpolation
+#endif
 constexpr int kRectChunk = 128; // views per refill of the rectangle table
 // Tile variants with 16-bit interpolation arithmetic (the plain format tags
 // CTR_F16 / CTR_BF16 widen the pairs and interpolate in FP32):
@@ -766,7 +776,7 @@ __device__ inline void lds_run_accumulate(float (&acc)[TY], const LdsRun &R, con
         // branch; the pairs are read as raw dwords and widened after the
         // scheduling barrier, so the reads of batch b + 1 stay in flight
         static_assert(DT == CTR_F16 || DT == CTR_BF16, "tile variant");
-        constexpr int kB = CTR_F32_BATCH;
+        constexpr int kB = DT == CTR_F16 ? CTR_WF16_BATCH : CTR_WBF16_BATCH;
         // 1.0 held in an SGPR: b * one - a on widened f16 halves then stays
         // one mixed-precision FMA (a literal 1.0 is folded into a
         // subtraction of two converted values)

@@ -273,8 +273,16 @@ def main():
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         log(f"CPU baseline on {threads} threads")
-        cpu = cpu_baseline(g, proj.cpu().numpy(), scale, threads, args.cpu_budget)
+        proj_cpu = proj.cpu().numpy()
+        cpu = cpu_baseline(g, proj_cpu, scale, threads, args.cpu_budget)
         log(f"  {cpu['value']:.3f} GUPS")
+        if threads > 1:
+            log("CPU baseline on 1 thread")
+            one = cpu_baseline(g, proj_cpu, scale, 1, min(6.0, args.cpu_budget / 2))
+            cpu["single_core"] = {"value": one["value"], "unit": "GUPS", "cores": 1,
+                                  "sample": one["sample"]}
+            log(f"  {one['value']:.4f} GUPS")
+        del proj_cpu
 
     out = {
         "metric": "FDK GUPS 512^3 x 720 views (FP16-stored filtered projections, FP32 "
