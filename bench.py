@@ -13,7 +13,9 @@ and the 16-bit forms once more with the 16-bit interpolation arithmetic
 Prints ONE JSON line.  Roofline for the dominant kernel (backprojection):
 algorithmic bytes per voxel-view update = 4 taps x element size + 8/N bytes
 of volume traffic, divided by the kernel's average duration (HIP events on
-the launch stream), against 8 TB/s.
+the launch stream), against 8 TB/s.  The taps are served from LDS, so that
+fraction exceeds 1; `traffic_frac` is the HBM bytes the counters measured per
+launch over the same duration, against the same peak.
 """
 import argparse
 import json
@@ -316,6 +318,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(ach16, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(ach16 / HBM_PEAK_GBPS, 4),
                      "traffic": traffic,
+                     "traffic_frac": (round(traffic / (bp_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
+                                      if traffic else None),
                      "kernel": "backproject_lds_kernel<fp16, 32> (FP32 interpolation; LDS 48 "
                                "columns, pitch 51, 72 rows)",
                      "avg_ms": round(bp_ms, 3)},
