@@ -182,6 +182,35 @@ def test_forward_project_vs_oracle(ctr, orc, dev, kind):
     assert torch.equal(sino.permute(1, 0, 2).cpu(), torch.from_numpy(got))
 
 
+@pytest.mark.parametrize("kind", ["cone", "parallel"])
+def test_forward_project_interior_vs_oracle(ctr, orc, dev, monkeypatch, kind):
+    """A volume large enough that most samples of a ray lie inside it and
+    take the unchecked interior loop (the samples near the faces still take
+    the checked one): against the oracle's ray-marcher, and the 64-bit
+    addressing path (used for volumes of 4 GiB and more) bit-identical to
+    the buffer-resource path."""
+    import torch
+    rng = np.random.default_rng(41)
+    X, Y, Z = 64, 48, 56
+    vol = rng.random((Z, Y, X)).astype(np.float32)
+    if kind == "cone":
+        g = ctr.Geometry(ctr.CONE, nu=80, nv=64, n_views=30, X=X, Y=Y, Z=Z,
+                         detector_width=160.0, volume_width=64.0, source_distance=150.0,
+                         detector_distance=100.0, rotation_offset=0.1)
+        ref = orc.forward_cone(vol, g.volume_width, g.source_distance, g.detector_distance,
+                               g.detector_width, g.nu, g.nv, g.n_views, 1.0, 0.1, 0.5)
+    else:
+        g = ctr.Geometry(ctr.PARALLEL, nu=96, nv=48, n_views=30, X=X, Y=Y, Z=Z,
+                         detector_width=96.0, volume_width=64.0, rotation_count=0.5)
+        ref = orc.forward_parallel(vol, g.volume_width, g.detector_width, g.nu, g.nv,
+                                   g.n_views, 0.5, 0.0, 0.5)
+    got = ctr.forward_project(_t(dev, vol), g, step=0.5)
+    assert rel_max(got.cpu().numpy(), ref) < 1e-4
+    monkeypatch.setenv("CTR_FORWARD_ADDRESSING", "pointer")
+    wide = ctr.forward_project(_t(dev, vol), g, step=0.5)
+    assert torch.equal(wide, got)
+
+
 def test_fdk_end_to_end_ball(ctr, orc, dev):
     """Analytic projections of a uniform ball -> FDK recovers its density."""
     g = ctr.Geometry(ctr.CONE, nu=128, nv=128, n_views=180, X=64, Y=64, Z=64,
