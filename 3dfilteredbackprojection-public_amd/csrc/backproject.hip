@@ -480,8 +480,9 @@ __global__ __launch_bounds__(256) void backproject_kernel(BpJob J)
 #endif
 // Voxels per software-pipelined batch of LDS reads (fp32 / packed fp16).
 // Measured on 512^3 x 720 and 1024^3 x 1200 (GUPS): fp32 batch 4 2767 /
-// 3254, batch 2 2792 / 3389, batch 8 2625 / 3071; packed fp16 without
-// pipelining 3343 / 3751, batch 2 3556 / 4146, batch 4 3487 / 4120.
+// 3254, batch 2 2792 / 3389, batch 8 2625 / 3071 (later, on the current
+// kernel: batch 1 2937 / 3635 against batch 2 2943 / 3641); packed fp16
+// without pipelining 3343 / 3751, batch 2 3556 / 4146, batch 4 3487 / 4120.
 #ifndef CTR_F32_BATCH
 #define CTR_F32_BATCH 2
 #endif
