@@ -1,11 +1,12 @@
 """The two interpolation modes of the 16-bit staged forms, through the C ABI.
 
-Under CTR_INTERP_FP32 the FP16 / BF16 pairs are widened and interpolated in
-FP32, so the only difference from the FP32 path is the storage rounding.
+Under CTR_INTERP_FP32 (the default) the FP16 / BF16 pairs are widened and
+interpolated in FP32, so the only difference from the FP32 path is the
+storage rounding.
 The reference here is the FP64 oracle backprojection of the DEQUANTIZED
 staged data (each stored 16-bit texel times its view scale, exact in FP64),
 which takes the storage rounding out: the FP32-interpolation result is held
-to the FP32 bar, max-rel <= 1e-4.  The default CTR_INTERP_FAST arithmetic is
+to the FP32 bar, max-rel <= 1e-4.  The opt-in CTR_INTERP_FAST arithmetic is
 checked against the same reference at the 16-bit forms' bars (FP16 1e-3,
 BF16 4e-3 rel RMSE), which isolates the interpolation arithmetic's own
 error, and must differ from the FP32-interpolation result (the switch takes
@@ -74,6 +75,32 @@ def test_fp32_interpolation_vs_dequantized_oracle(ctr, orc, dev, interp, X, Y, Z
         assert e_strict < 1e-4, (dt, e_strict)
         assert e_fast < tol, (dt, e_fast)
         assert not np.array_equal(strict, fast), dt
+
+
+def test_fp32_interpolation_parallel_vs_dequantized_oracle(ctr, orc, dev, interp):
+    """Parallel-beam volume (the LDS kernel: 40 voxels along y) under both
+    modes, against the FP64 oracle of the dequantized staged data."""
+    import torch
+    g = ctr.Geometry(ctr.PARALLEL, nu=64, nv=48, n_views=100, X=40, Y=40, Z=36,
+                     detector_width=64.0, volume_width=40.0, rotation_count=0.5,
+                     rotation_offset=0.15)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(77)
+    p = torch.randn((g.n_views, g.nv, g.nu), generator=gen, device=dev)
+    for dt, tol in ((ctr.F16, 1e-3), (ctr.BF16, 4e-3)):
+        staged, vs = ctr.stage_projections(p, g, dt)
+        q = dequantize(ctr, staged, vs, g, dt)
+        ref = orc.backproject_parallel(q, g.detector_width, g.volume_width, g.rotation_count,
+                                       g.rotation_offset, 0.02, (g.X, g.Y, g.Z))
+        interp(ctr.INTERP_FP32)
+        strict = ctr.backproject(staged, g, dt, view_scale=vs, scale=0.02).cpu().numpy()
+        interp(ctr.INTERP_FAST)
+        fast = ctr.backproject(staged, g, dt, view_scale=vs, scale=0.02).cpu().numpy()
+        e_strict, e_fast = rel_max(strict, ref), rel_rmse(fast, ref)
+        print(f"[interp] parallel dt={dt}: fp32 max-rel {e_strict:.2e}, "
+              f"fast rel-rmse {e_fast:.2e}", flush=True)
+        assert e_strict < 1e-4, (dt, e_strict)
+        assert e_fast < tol, (dt, e_fast)
 
 
 def test_packed_follows_interpolation_mode(ctr, dev, interp):
