@@ -254,8 +254,9 @@ int ctr_backproject_packed(void *stream, ctr_workspace *ws, int staged_dtype,
  * weighted in FP32) or CTR_INTERP_FAST (the 16-bit arithmetic described at
  * ctr_backproject: 10-15% faster, larger error).  The initial mode is
  * CTR_INTERP_FAST when the environment variable CTR_INTERPOLATION is "fast"
- * at the first call, else CTR_INTERP_FP32.  ctr_set_interpolation returns
- * the previous mode, or
+ * at the first call, else CTR_INTERP_FP32.  The mode picks the kernel at
+ * each launch: a HIP graph captured before a change keeps replaying the
+ * kernel it captured.  ctr_set_interpolation returns the previous mode, or
  * CTR_ERR_INVALID_ARGUMENT for an unknown one.  No reference counterpart
  * (the reference interpolates FP32 texels in the sampler). */
 int ctr_set_interpolation(int mode);
