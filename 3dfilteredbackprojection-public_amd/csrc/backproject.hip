@@ -478,7 +478,9 @@ __global__ __launch_bounds__(256) void backproject_kernel(BpJob J)
 // Voxels per lane along y.  24 (16-bit rows 54 or 56: 6 workgroups per CU
 // instead of 5) measured fp16 3194-3213 / 3692 / 3373-3564 GUPS at 512^3 /
 // 1024^3 / 1440^3 against 3366 / 3853 / 3962 for 32: the per-view set-up,
-// shared by fewer voxels, costs more than the sixth workgroup gains.
+// shared by fewer voxels, costs more than the sixth workgroup gains.  40
+// (16-bit rows 80 or 88: 4 workgroups per CU) measured fp16 3256-3363 /
+// 3757-3786 / 3586-3757 against 3391 / 3819 / 3948 for 32.
 #ifndef CTR_LDS_TY
 #define CTR_LDS_TY 32
 #endif
