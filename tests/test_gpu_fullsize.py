@@ -114,18 +114,20 @@ def test_c3_storage_forms_vs_oracle(ctr, orc, dev):
              ("fp16/fast", ctr.F16), ("bf16/fast", ctr.BF16))
     got, filtered = {}, None
     start = ctr.get_interpolation()
-    for name, dt in forms:
-        ctr.set_interpolation(ctr.INTERP_FAST if name.endswith("fast") else ctr.INTERP_FP32)
-        staged, vs = ctr.filter_projections(proj, g, ctr.RAM_LAK, 4096, dt, fdk=True)
-        vol = ctr.backproject(staged, g, dt, view_scale=vs, scale=scale)
-        got[name] = _crop(vol, box)
-        if dt == ctr.F32:
-            pu, pv, _ = ctr.staged_layout(g.nu, g.nv, g.n_views)
-            st = staged.view(torch.float32).view(g.n_views, pv, pu)
-            filtered = st[:, 1:g.nv + 1, 1:g.nu + 1].cpu().numpy()
-        del staged, vol
-        torch.cuda.synchronize()
-    ctr.set_interpolation(start)
+    try:
+        for name, dt in forms:
+            ctr.set_interpolation(ctr.INTERP_FAST if name.endswith("fast") else ctr.INTERP_FP32)
+            staged, vs = ctr.filter_projections(proj, g, ctr.RAM_LAK, 4096, dt, fdk=True)
+            vol = ctr.backproject(staged, g, dt, view_scale=vs, scale=scale)
+            got[name] = _crop(vol, box)
+            if dt == ctr.F32:
+                pu, pv, _ = ctr.staged_layout(g.nu, g.nv, g.n_views)
+                st = staged.view(torch.float32).view(g.n_views, pv, pu)
+                filtered = st[:, 1:g.nv + 1, 1:g.nu + 1].cpu().numpy()
+            del staged, vol
+            torch.cuda.synchronize()
+    finally:
+        ctr.set_interpolation(start)
     del proj
     ref = orc.backproject_cone_region(filtered, g.source_distance, g.detector_distance,
                                       g.detector_width, g.volume_width, 1.0, 0.0, scale,
