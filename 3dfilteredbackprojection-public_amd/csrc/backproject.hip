@@ -979,8 +979,30 @@ __device__ inline int xcd_tile_order(int b, int nblocks)
     return xcd * q + min(xcd, r) + k;
 }
 
+// Waves per SIMD the compiler must leave registers for (the second launch
+// bound; one wave of each workgroup per SIMD, so also workgroups per CU), fp32
+// and 16-bit tiles.  1 leaves the register budget to the compiler.  Measured
+// (GUPS, 512^3 / 1024^3 / 1440^3): fp32 at 5 (74 VGPRs in place of 120) with
+// 70 rows of pitch 53 2920 / 3708 / 3610, with 72 rows of pitch 51 2877 /
+// 3685 / 3697, against 2950 / 3621 / 3763 at 4 workgroups (84 rows): the
+// taller footprints that then take the direct pass cost more than the fifth
+// workgroup gains.  16-bit at 6 (64-68 VGPRs, 60 rows) 2564 / 3840 / 2577
+// against 3414 / 3843 / 3956 for fp16: no gain even where every footprint
+// fits (1024^3).
+#ifndef CTR_F32_MINB
+#define CTR_F32_MINB 1
+#endif
+#ifndef CTR_16_MINB
+#define CTR_16_MINB 1
+#endif
+template <int DT>
+constexpr int lds_min_blocks()
+{
+    return DT == CTR_F32 ? CTR_F32_MINB : CTR_16_MINB;
+}
+
 template <int DT, int TY>
-__global__ __launch_bounds__(256) void backproject_lds_kernel(BpJob J)
+__global__ __launch_bounds__(256, lds_min_blocks<DT>()) void backproject_lds_kernel(BpJob J)
 {
     __shared__ __align__(16) unsigned lds[LdsLayout<DT>::kDwords];
     const int t = xcd_tile_order(blockIdx.x, gridDim.x);
@@ -993,7 +1015,8 @@ __global__ __launch_bounds__(256) void backproject_lds_kernel(BpJob J)
 // Packed launch: the flat tile index selects the job (the last job whose
 // block_begin it reaches) and the tile inside it.
 template <int DT, int TY>
-__global__ __launch_bounds__(256) void backproject_lds_packed_kernel(BpBatch B)
+__global__ __launch_bounds__(256, lds_min_blocks<DT>()) void backproject_lds_packed_kernel(
+    BpBatch B)
 {
     __shared__ __align__(16) unsigned lds[LdsLayout<DT>::kDwords];
     const int b = xcd_tile_order(blockIdx.x, gridDim.x);
