@@ -108,3 +108,31 @@ def test_interpolation_mode_from_environment(ctr, value, mode):
     out = subprocess.run(["python", "-c", code, ctr.LIB_PATH], env=env,
                          capture_output=True, text=True, check=True).stdout
     assert int(out.strip()) == mode
+
+
+def test_size_limits_without_device(ctr):
+    """Degenerate and over-limit sizes are rejected on the host, before any
+    device call: empty view sets, sizes of 2^20 and more, a staged view past
+    the backprojector's 2^22-texel offsets, packed counts outside 1..16 and a
+    non-positive forward step."""
+    lib = ctr.lib()
+    fake = ctypes.c_void_p(64)
+    bp = lambda nu, nv, n, X, dt=0, vs=None: lib.ctr_backproject(
+        None, None, 0, fake, dt, vs, nu, nv, n, 500.0, 500.0, 600.0, 200.0, 1.0, 0.0,
+        fake, X, X, X, 1.0, 0, 0)
+    assert bp(64, 64, 0, 32) == -1
+    assert "sizes must be positive" in ctr.last_error()
+    assert bp(64, 64, 8, 1 << 20) == -1
+    assert "below 2^20" in ctr.last_error()
+    # 2100 x 2100 detector: a staged view of 2112 x 2103 texels > 2^22
+    assert bp(2100, 2100, 8, 32) == -1
+    assert "2^22" in ctr.last_error()
+    assert bp(2100, 2100, 8, 32, dt=1, vs=fake) == -1
+    jobs = (ctr.BackprojectJob * 17)()
+    assert lib.ctr_backproject_packed(None, None, 0, jobs, 0) == -1
+    assert lib.ctr_backproject_packed(None, None, 0, jobs, 17) == -1
+    assert lib.ctr_backproject_packed(None, None, 0, None, 1) == -5
+    assert lib.ctr_backproject_packed(None, None, 7, jobs, 1) == -1
+    assert lib.ctr_forward_project(None, 0, fake, 8, 8, 8, 100.0, 1.0, 10.0, 10.0, 1.0,
+                                   0.0, fake, 0, 8, 8, 4, 0.0) == -1
+    assert "step must be positive" in ctr.last_error()
