@@ -969,15 +969,8 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
     store_run<TY>(acc, J.vol, i, j0, k, J.X, J.Y, J.Z, J.accumulate, J.clip);
 }
 
-// Workgroups are dispatched to the 8 XCDs round-robin; this renumbering hands
-// each XCD one contiguous range of tiles, so neighbouring tiles -- which
-// read overlapping detector rows -- share that XCD's L2.
-__device__ inline int xcd_tile_order(int b, int nblocks)
-{
-    const int xcd = b & 7, k = b >> 3;
-    const int q = nblocks >> 3, r = nblocks & 7;
-    return xcd * q + min(xcd, r) + k;
-}
+// Tiles reach the XCDs through xcd_tile_order: neighbouring tiles, which read
+// overlapping detector rows, share one XCD's L2.
 
 // Waves per SIMD the compiler must leave registers for (the second launch
 // bound; one wave of each workgroup per SIMD, so also workgroups per CU), fp32

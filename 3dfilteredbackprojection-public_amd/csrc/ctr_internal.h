@@ -89,4 +89,14 @@ int check_geometry(int geometry, double sod, double odd, double detector_width,
                    double volume_width, int X, int Y, int Z, int nu, int nv,
                    int n_views);
 
+// Workgroups are dispatched to the 8 XCDs round-robin; this renumbering of a
+// 1D grid hands each XCD one contiguous range of block indices, so blocks
+// that read neighbouring data share that XCD's L2.
+__device__ inline int xcd_tile_order(int b, int nblocks)
+{
+    const int xcd = b & 7, k = b >> 3;
+    const int q = nblocks >> 3, r = nblocks & 7;
+    return xcd * q + min(xcd, r) + k;
+}
+
 }  // namespace ctr
