@@ -167,36 +167,19 @@ __device__ inline float trilinear_interior_ptr(const float *__restrict__ vol, in
 // contiguous range of views.  Measured (ms, 512^3 / 256^3 / 1440^3): 0 1164
 // / 53.1 / 20642, 1 1272 / 68.5 / 26997 -- with every XCD on the same view
 // the volume streams through the shared last-level cache once per view.
-#ifndef CTR_FWD_XCD
-#define CTR_FWD_XCD 0
-#endif
-// Detector pixels of a 256-thread block along u (the rest along v): compact
-// tiles keep the block's rays close in the volume, so their gathers share
-// cache lines.  Measured (ms, 512^3 / 256^3 inputs): 4 wide 1234 / 58.1,
-// 8 wide 1184 / 52.0, 16 wide 1160 / 52.9, 32 wide 1284 / 58.6, 64 wide
-// 1626 / 70.5.
-#ifndef CTR_FWD_BU
-#define CTR_FWD_BU 16
-#endif
-#ifndef CTR_FWD_UNROLL_BUF
-#define CTR_FWD_UNROLL_BUF 1
-#endif
-#ifndef CTR_FWD_UNROLL_PTR
-#define CTR_FWD_UNROLL_PTR 2
-#endif
+// Index-space ray of detector pixel (iu, iv) in view n: sample k < K sits at
+// voxel-index coordinates b + (k + 1/2) s (x, y, z), the ray clipped to the
+// volume box grown by one voxel per side; K = 0 when it misses the box.  Set
+// up in FP64 (FBP3D.cpp:280-284 angle convention), stepped in FP32.
+struct IndexRay {
+    float b[3], s[3];
+    int K;
+};
 
-template <bool kBuf>
-__global__ __launch_bounds__(256) void forward_kernel(ForwardJob J)
+__device__ inline IndexRay index_ray(const ForwardJob &J, int n, int iu, int iv)
 {
-    // 1D grid of (view, v tile, u tile)
-    const int tu = (J.nu + CTR_FWD_BU - 1) / CTR_FWD_BU;
-    const int tv = (J.nv + 256 / CTR_FWD_BU - 1) / (256 / CTR_FWD_BU);
-    const int b = CTR_FWD_XCD ? xcd_tile_order(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    const int n = b / (tu * tv), tile = b % (tu * tv);
-    const int iu = (tile % tu) * CTR_FWD_BU + (threadIdx.x % CTR_FWD_BU);
-    const int iv = (tile / tu) * (256 / CTR_FWD_BU) + (threadIdx.x / CTR_FWD_BU);
-    if (iu >= J.nu || iv >= J.nv) return;
-    // ray set-up in FP64 (FBP3D.cpp:280-284 angle convention)
+    IndexRay R;
+    R.K = 0;
     double th = 2.0 * M_PI * J.rot * (double)n / (double)J.n_views + J.offset;
     double s, c;
     sincos(th, &s, &c);
@@ -240,17 +223,55 @@ __global__ __launch_bounds__(256) void forward_kernel(ForwardJob J)
         tmin = fmax(tmin, t0);
         tmax = fmin(tmax, t1);
     }
-    float acc = 0.0f;
+    for (int a = 0; a < 3; a++) R.b[a] = R.s[a] = 0.0f;
     if (hit && tmax > tmin) {
-        const int K = (int)ceil((tmax - tmin) / J.h);
-        // index-space ray: f(t') = base + t' * step with t' = k + 1/2
+        R.K = (int)ceil((tmax - tmin) / J.h);
         const double inv_d = 1.0 / J.d;
-        const float bi = (float)((o[0] + tmin * dir[0]) * inv_d + 0.5 * J.X - 0.5);
-        const float bj = (float)((o[1] + tmin * dir[1]) * inv_d + 0.5 * J.Y - 0.5);
-        const float bk = (float)((o[2] + tmin * dir[2]) * inv_d + 0.5 * J.Z - 0.5);
-        const float si = (float)(dir[0] * J.h * inv_d);
-        const float sj = (float)(dir[1] * J.h * inv_d);
-        const float sk = (float)(dir[2] * J.h * inv_d);
+        R.b[0] = (float)((o[0] + tmin * dir[0]) * inv_d + 0.5 * J.X - 0.5);
+        R.b[1] = (float)((o[1] + tmin * dir[1]) * inv_d + 0.5 * J.Y - 0.5);
+        R.b[2] = (float)((o[2] + tmin * dir[2]) * inv_d + 0.5 * J.Z - 0.5);
+        R.s[0] = (float)(dir[0] * J.h * inv_d);
+        R.s[1] = (float)(dir[1] * J.h * inv_d);
+        R.s[2] = (float)(dir[2] * J.h * inv_d);
+    }
+    return R;
+}
+
+#ifndef CTR_FWD_XCD
+#define CTR_FWD_XCD 0
+#endif
+// Detector pixels of a 256-thread block along u (the rest along v): compact
+// tiles keep the block's rays close in the volume, so their gathers share
+// cache lines.  Measured (ms, 512^3 / 256^3 inputs): 4 wide 1234 / 58.1,
+// 8 wide 1184 / 52.0, 16 wide 1160 / 52.9, 32 wide 1284 / 58.6, 64 wide
+// 1626 / 70.5.
+#ifndef CTR_FWD_BU
+#define CTR_FWD_BU 16
+#endif
+#ifndef CTR_FWD_UNROLL_BUF
+#define CTR_FWD_UNROLL_BUF 1
+#endif
+#ifndef CTR_FWD_UNROLL_PTR
+#define CTR_FWD_UNROLL_PTR 2
+#endif
+
+template <bool kBuf>
+__global__ __launch_bounds__(256) void forward_kernel(ForwardJob J)
+{
+    // 1D grid of (view, v tile, u tile)
+    const int tu = (J.nu + CTR_FWD_BU - 1) / CTR_FWD_BU;
+    const int tv = (J.nv + 256 / CTR_FWD_BU - 1) / (256 / CTR_FWD_BU);
+    const int b = CTR_FWD_XCD ? xcd_tile_order(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int n = b / (tu * tv), tile = b % (tu * tv);
+    const int iu = (tile % tu) * CTR_FWD_BU + (threadIdx.x % CTR_FWD_BU);
+    const int iv = (tile / tu) * (256 / CTR_FWD_BU) + (threadIdx.x / CTR_FWD_BU);
+    if (iu >= J.nu || iv >= J.nv) return;
+    const IndexRay R = index_ray(J, n, iu, iv);
+    float acc = 0.0f;
+    if (R.K > 0) {
+        const int K = R.K;
+        const float bi = R.b[0], bj = R.b[1], bk = R.b[2];
+        const float si = R.s[0], sj = R.s[1], sk = R.s[2];
         // samples in order: checked ones before the interior range, the
         // interior ones without bounds tests, checked ones after it (the sum
         // is the same sequence of additions as one checked loop)
@@ -293,6 +314,219 @@ __global__ __launch_bounds__(256) void forward_kernel(ForwardJob J)
     size_t idx = J.layout == CTR_LAYOUT_PROJECTIONS
                      ? ((size_t)n * J.nv + iv) * J.nu + iu
                      : ((size_t)iv * J.n_views + n) * J.nu + iu;
+    J.out[idx] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Slab-staged forward projection.  The block's 256 rays (one 16 x 16 detector
+// tile of one view) march together through slabs of kSlabS slices along the
+// view's main horizontal axis A (x when |cos theta| >= |sin theta|, else z),
+// consecutive slabs sharing one slice.  For each slab the block stages the
+// box of voxels its rays' samples can touch there -- zero outside the volume
+// -- in LDS, and every ray takes the samples whose cell starts in the slab
+// from LDS.  Each ray still sums its samples in order with trilinear0's
+// arithmetic on the same taps, so the result equals forward_kernel's bit for
+// bit.  A slab whose box exceeds the LDS region reads the volume directly, and
+// a ray that does not run along +A (or -A) like the view's central ray, or a
+// sample outside every slab, falls back to trilinear0 after the slabs, in
+// order.
+// ---------------------------------------------------------------------------
+// Slab depth and box capacity.  Measured (ms, 512^3 / 256^3 inputs, all
+// bit-identical): 4 slices / 12 KB 837 / 51.4, 5 / 15 KB 696 / 42.4, 6 /
+// 18 KB 654 / 40.5, 8 / 20 KB 676 / 41.5, 8 / 24 KB 718 / 43.9, 8 / 36 KB
+// 919 / 54.5, 12 / 36 KB 831 / 48.8, 16 / 48 KB 1158 / 68.6; the ray kernel
+// 1163 / 53.1.  Up to 20 KB eight workgroups fit a CU.
+#ifndef CTR_FWD_SLAB_S
+#define CTR_FWD_SLAB_S 6
+#endif
+#ifndef CTR_FWD_SLAB_CAP
+#define CTR_FWD_SLAB_CAP 4608
+#endif
+// 1: each XCD takes one contiguous eighth of every view's tiles (neighbouring
+// boxes on one L2).  Measured 700 / 43.2 ms against 655 / 40.4 ms for the
+// plain order (512^3 / 256^3 inputs), so it stays off.
+#ifndef CTR_FWD_SLAB_XCD
+#define CTR_FWD_SLAB_XCD 0
+#endif
+constexpr int kSlabS = CTR_FWD_SLAB_S;      // slices per slab
+constexpr int kSlabCap = CTR_FWD_SLAB_CAP;  // floats of the staged box
+
+__device__ inline int wave_min(int v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+    return v;
+}
+__device__ inline int wave_max(int v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+    return v;
+}
+
+template <bool kBuf>
+__global__ __launch_bounds__(256) void forward_slab_kernel(ForwardJob J)
+{
+    __shared__ float box[kSlabCap];
+    // per-wave partial bounds, double-buffered by slab parity: a slab that no
+    // ray samples skips its second barrier, so a wave may write the next
+    // slab's partials while another still reads this slab's
+    __shared__ int red[2][4][4];
+    const int tu = (J.nu + CTR_FWD_BU - 1) / CTR_FWD_BU;
+    const int tv = (J.nv + 256 / CTR_FWD_BU - 1) / (256 / CTR_FWD_BU);
+    const int P = tu * tv;
+    int n, tile;
+    if (CTR_FWD_SLAB_XCD) {
+        // blocks reach the 8 XCDs round-robin: XCD x takes tiles x P8 ..
+        // x P8 + P8 - 1 of every view, all XCDs on the same view at a time
+        // (the grid holds 8 P8 blocks per view; the padding ones return)
+        const int P8 = (P + 7) / 8;
+        const int x = blockIdx.x & 7, kk = blockIdx.x >> 3;
+        n = kk / P8;
+        tile = x * P8 + kk % P8;
+        if (tile >= P) return;
+    } else {
+        n = blockIdx.x / P;
+        tile = blockIdx.x % P;
+    }
+    const int iu = (tile % tu) * CTR_FWD_BU + (threadIdx.x % CTR_FWD_BU);
+    const int iv = (tile / tu) * (256 / CTR_FWD_BU) + (threadIdx.x / CTR_FWD_BU);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bool pix = iu < J.nu && iv < J.nv;
+    IndexRay R;
+    R.K = 0;
+    for (int a = 0; a < 3; a++) R.b[a] = R.s[a] = 0.0f;
+    if (pix) R = index_ray(J, n, iu, iv);
+
+    // the view's main horizontal axis and the central ray's direction on it
+    const double th = 2.0 * M_PI * J.rot * (double)n / (double)J.n_views + J.offset;
+    double sn, cs;
+    sincos(th, &sn, &cs);
+    const bool ax_x = fabs(cs) >= fabs(sn);
+    const bool up = ax_x ? cs < 0.0 : sn < 0.0;
+    const int dimA = ax_x ? J.X : J.Z;
+    const float bA = ax_x ? R.b[0] : R.b[2], sA = ax_x ? R.s[0] : R.s[2];
+    const float bC = ax_x ? R.b[2] : R.b[0], sC = ax_x ? R.s[2] : R.s[0];
+    const bool along = up ? sA > 0.0f : sA < 0.0f;  // else: all samples after the slabs
+
+    // box fills through a buffer resource below 4 GiB, 64-bit addresses above
+    const unsigned vol_bytes = kBuf ? (unsigned)J.X * (unsigned)J.Y * (unsigned)J.Z * 4u : 0u;
+    const __amdgpu_buffer_rsrc_t vrs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float *>(J.vol), (short)0, (int)vol_bytes, 0x00020000);
+
+    // cells start at index -2 .. dimA along A (samples lie in the box grown
+    // by one voxel); slab m holds cell starts lo .. lo + S - 2, slices lo ..
+    // lo + S - 1, lo = -2 + m (S - 1)
+    constexpr int S = kSlabS;
+    const int M = (dimA + 3 + S - 2) / (S - 1);
+    float acc = 0.0f;
+    int k = 0;
+#pragma unroll 1
+    for (int mm = 0; mm < M; ++mm) {
+        const int m = up ? mm : M - 1 - mm;
+        const int lo = -2 + m * (S - 1), hi = lo + S - 2;
+        // this ray's samples in the slab: t = k + 1/2 with lo <= f_A(t) < hi + 1;
+        // their y and C cells, widened by one voxel for rounding
+        int yl = 1 << 30, yh = -(1 << 30), cl = 1 << 30, ch = -(1 << 30);
+        if (along && k < R.K) {
+            float ta = ((float)lo - bA) / sA, tb = ((float)(hi + 1) - bA) / sA;
+            if (ta > tb) {
+                const float t = ta;
+                ta = tb;
+                tb = t;
+            }
+            ta = fmaxf(ta, (float)k + 0.5f) - 0.5f;
+            tb = fminf(tb, (float)R.K - 0.5f) + 0.5f;
+            if (ta <= tb) {
+                const float y0 = fmaf(ta, R.s[1], R.b[1]), y1 = fmaf(tb, R.s[1], R.b[1]);
+                const float c0 = fmaf(ta, sC, bC), c1 = fmaf(tb, sC, bC);
+                yl = (int)floorf(fminf(y0, y1)) - 1;
+                yh = (int)floorf(fmaxf(y0, y1)) + 2;
+                cl = (int)floorf(fminf(c0, c1)) - 1;
+                ch = (int)floorf(fmaxf(c0, c1)) + 2;
+            }
+        }
+        yl = wave_min(yl);
+        yh = wave_max(yh);
+        cl = wave_min(cl);
+        ch = wave_max(ch);
+        int(*rd)[4] = red[mm & 1];
+        if (lane == 0) {
+            rd[wave][0] = yl;
+            rd[wave][1] = yh;
+            rd[wave][2] = cl;
+            rd[wave][3] = ch;
+        }
+        __syncthreads();  // also: every ray is done with the previous slab's box
+        yl = min(min(rd[0][0], rd[1][0]), min(rd[2][0], rd[3][0]));
+        yh = max(max(rd[0][1], rd[1][1]), max(rd[2][1], rd[3][1]));
+        cl = min(min(rd[0][2], rd[1][2]), min(rd[2][2], rd[3][2]));
+        ch = max(max(rd[0][3], rd[1][3]), max(rd[2][3], rd[3][3]));
+        if (yh < yl) continue;  // no ray samples this slab (block-uniform)
+        const int Ry = yh - yl + 1, Rc = ch - cl + 1;
+        const bool staged = (long long)S * Ry * Rc <= kSlabCap;
+        // box layout, x fastest: A = x: [C][y][A], A = z: [A][y][C]
+        const int sy = ax_x ? S : Rc, sz = ax_x ? S * Ry : Rc * Ry;
+        const int x0 = ax_x ? lo : cl, z0 = ax_x ? cl : lo;
+        const int nx = ax_x ? S : Rc, nz = ax_x ? Rc : S;
+        if (staged) {
+            const int total = S * Ry * Rc;
+            for (int e = threadIdx.x; e < total; e += 256) {
+                // e = (z' Ry + y') nx + x'
+                const int xr = e % nx, r = e / nx;
+                const int yr = r % Ry, zr = r / Ry;
+                const int x = x0 + xr, y = yl + yr, z = z0 + zr;
+                float v = 0.0f;
+                if (x >= 0 && x < J.X && y >= 0 && y < J.Y && z >= 0 && z < J.Z) {
+                    if constexpr (kBuf)
+                        v = __builtin_bit_cast(
+                            float, __builtin_amdgcn_raw_buffer_load_b32(
+                                       vrs, (unsigned)((z * J.Y + y) * J.X + x) * 4u, 0, 0));
+                    else
+                        v = J.vol[((size_t)z * J.Y + y) * J.X + x];
+                }
+                box[e] = v;
+            }
+            (void)nz;
+        }
+        __syncthreads();
+        if (!along) continue;
+#pragma unroll 1
+        while (k < R.K) {
+            const float t = (float)k + 0.5f;
+            const float fi = fmaf(t, R.s[0], R.b[0]), fj = fmaf(t, R.s[1], R.b[1]),
+                        fk = fmaf(t, R.s[2], R.b[2]);
+            const int ia = (int)floorf(ax_x ? fi : fk);
+            if (ia < lo || ia > hi) break;
+            if (staged) {
+                // trilinear0 on the staged box (zero outside the volume)
+                const float i0f = floorf(fi), j0f = floorf(fj), k0f = floorf(fk);
+                const int i0 = (int)i0f, j0 = (int)j0f, k0 = (int)k0f;
+                const float wi = fi - i0f, wj = fj - j0f, wk = fk - k0f;
+                int base = (i0 - x0) + (j0 - yl) * sy + (k0 - z0) * sz;
+                base = min(max(base, 0), S * Ry * Rc - 2 - sy - sz);
+                const float *p = box + base;
+                const float c00 = fmaf(wi, p[1] - p[0], p[0]);
+                const float c01 = fmaf(wi, p[sy + 1] - p[sy], p[sy]);
+                const float c10 = fmaf(wi, p[sz + 1] - p[sz], p[sz]);
+                const float c11 = fmaf(wi, p[sz + sy + 1] - p[sz + sy], p[sz + sy]);
+                const float c0 = fmaf(wj, c01 - c00, c00);
+                const float c1 = fmaf(wj, c11 - c10, c10);
+                acc += fmaf(wk, c1 - c0, c0);
+            } else {
+                acc += trilinear0(J.vol, J.X, J.Y, J.Z, fi, fj, fk);
+            }
+            ++k;
+        }
+    }
+    for (; k < R.K; ++k) {
+        const float t = (float)k + 0.5f;
+        acc += trilinear0(J.vol, J.X, J.Y, J.Z, fmaf(t, R.s[0], R.b[0]),
+                          fmaf(t, R.s[1], R.b[1]), fmaf(t, R.s[2], R.b[2]));
+    }
+    if (!pix) return;
+    if (R.K > 0) acc *= (float)J.h;
+    const size_t idx = J.layout == CTR_LAYOUT_PROJECTIONS
+                           ? ((size_t)n * J.nv + iv) * J.nu + iu
+                           : ((size_t)iv * J.n_views + n) * J.nu + iu;
     J.out[idx] = acc;
 }
 
@@ -514,7 +748,21 @@ int ctr_forward_project(void *stream, int geometry, const float *volume, int X, 
     const bool buf = bytes <= 0xfffffff0ull && (unsigned long long)X * Y < (1ull << 24) &&
                      X < (1 << 24) && Y < (1 << 24) && Z < (1 << 24) &&
                      !(force && !strcmp(force, "pointer"));
-    if (buf)
+    // the slab-staged kernel by default, its boxes filled through the same
+    // buffer resource or 64-bit addresses (CTR_FORWARD_KERNEL=ray forces the
+    // ray kernel: diagnostics and tests)
+    const char *kern = getenv("CTR_FORWARD_KERNEL");
+    const bool slab = !(kern && !strcmp(kern, "ray"));
+    const long long tiles = blocks / n_views;
+    const dim3 slab_grid(
+        (unsigned)(CTR_FWD_SLAB_XCD ? 8 * ((tiles + 7) / 8) * n_views : blocks));
+    CTR_REQUIRE((long long)slab_grid.x < (1LL << 31), CTR_ERR_INVALID_ARGUMENT,
+                "forward projection too large");
+    if (slab && buf)
+        forward_slab_kernel<true><<<slab_grid, 256, 0, as_stream(stream)>>>(J);
+    else if (slab)
+        forward_slab_kernel<false><<<slab_grid, 256, 0, as_stream(stream)>>>(J);
+    else if (buf)
         forward_kernel<true><<<grid, 256, 0, as_stream(stream)>>>(J);
     else
         forward_kernel<false><<<grid, 256, 0, as_stream(stream)>>>(J);

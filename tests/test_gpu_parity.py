@@ -186,9 +186,9 @@ def test_forward_project_vs_oracle(ctr, orc, dev, kind):
 def test_forward_project_interior_vs_oracle(ctr, orc, dev, monkeypatch, kind):
     """A volume large enough that most samples of a ray lie inside it and
     take the unchecked interior loop (the samples near the faces still take
-    the checked one): against the oracle's ray-marcher, and the 64-bit
-    addressing path (used for volumes of 4 GiB and more) bit-identical to
-    the buffer-resource path."""
+    the checked one): against the oracle's ray-marcher, and the slab-staged
+    kernel (the default) and the ray kernel, each with buffer-resource
+    addressing (below 4 GiB) and 64-bit addressing, bit-identical."""
     import torch
     rng = np.random.default_rng(41)
     X, Y, Z = 64, 48, 56
@@ -206,9 +206,48 @@ def test_forward_project_interior_vs_oracle(ctr, orc, dev, monkeypatch, kind):
                                    g.n_views, 0.5, 0.0, 0.5)
     got = ctr.forward_project(_t(dev, vol), g, step=0.5)
     assert rel_max(got.cpu().numpy(), ref) < 1e-4
+    monkeypatch.setenv("CTR_FORWARD_KERNEL", "ray")
+    ray = ctr.forward_project(_t(dev, vol), g, step=0.5)
+    assert torch.equal(ray, got)
     monkeypatch.setenv("CTR_FORWARD_ADDRESSING", "pointer")
     wide = ctr.forward_project(_t(dev, vol), g, step=0.5)
     assert torch.equal(wide, got)
+    monkeypatch.delenv("CTR_FORWARD_KERNEL")
+    wide_slab = ctr.forward_project(_t(dev, vol), g, step=0.5)
+    assert torch.equal(wide_slab, got)
+
+
+@pytest.mark.parametrize("step", [0.5, 1.0, 2.0])
+def test_forward_slab_kernel_matches_ray_kernel(ctr, dev, monkeypatch, step):
+    """The slab-staged forward projector against the ray kernel, bit for bit:
+    a full turn of views (every main axis and direction), a fan wide enough
+    that some rays run against the view's central direction (those take the
+    in-order fallback), odd sizes and several steps."""
+    import torch
+    rng = np.random.default_rng(int(step * 10))
+    X, Y, Z = 45, 37, 51
+    vol = _t(dev, rng.random((Z, Y, X)).astype(np.float32))
+    geoms = [ctr.Geometry(ctr.CONE, nu=70, nv=50, n_views=37, X=X, Y=Y, Z=Z,
+                          detector_width=140.0, volume_width=45.0, source_distance=120.0,
+                          detector_distance=90.0, rotation_offset=0.05),
+             ctr.Geometry(ctr.CONE, nu=90, nv=30, n_views=24, X=X, Y=Y, Z=Z,
+                          detector_width=600.0, volume_width=45.0, source_distance=60.0,
+                          detector_distance=40.0, rotation_offset=0.3),
+             ctr.Geometry(ctr.PARALLEL, nu=64, nv=40, n_views=29, X=X, Y=Y, Z=Z,
+                          detector_width=64.0, volume_width=45.0, rotation_count=0.5,
+                          rotation_offset=0.2)]
+    for g in geoms:
+        monkeypatch.delenv("CTR_FORWARD_KERNEL", raising=False)
+        monkeypatch.delenv("CTR_FORWARD_ADDRESSING", raising=False)
+        slab = ctr.forward_project(vol, g, step=step)
+        monkeypatch.setenv("CTR_FORWARD_ADDRESSING", "pointer")
+        slab_wide = ctr.forward_project(vol, g, step=step)
+        monkeypatch.setenv("CTR_FORWARD_KERNEL", "ray")
+        monkeypatch.delenv("CTR_FORWARD_ADDRESSING")
+        ray = ctr.forward_project(vol, g, step=step)
+        assert float(ray.abs().max()) > 0.0
+        assert torch.equal(slab, ray), (g.kind, g.nu, step)
+        assert torch.equal(slab_wide, ray), (g.kind, g.nu, step)
 
 
 def test_fdk_end_to_end_ball(ctr, orc, dev):
