@@ -466,7 +466,7 @@ __global__ __launch_bounds__(256) void forward_slab_kernel(ForwardJob J)
         // box layout, x fastest: A = x: [C][y][A], A = z: [A][y][C]
         const int sy = ax_x ? S : Rc, sz = ax_x ? S * Ry : Rc * Ry;
         const int x0 = ax_x ? lo : cl, z0 = ax_x ? cl : lo;
-        const int nx = ax_x ? S : Rc, nz = ax_x ? Rc : S;
+        const int nx = ax_x ? S : Rc;
         if (staged) {
             const int total = S * Ry * Rc;
             for (int e = threadIdx.x; e < total; e += 256) {
@@ -485,7 +485,6 @@ __global__ __launch_bounds__(256) void forward_slab_kernel(ForwardJob J)
                 }
                 box[e] = v;
             }
-            (void)nz;
         }
         __syncthreads();
         if (!along) continue;
