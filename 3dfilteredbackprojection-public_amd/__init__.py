@@ -23,6 +23,9 @@ LIB_PATH = os.path.join(HERE, "libctrecon.so")
 CTR_OK = 0
 F32, F16, BF16 = 0, 1, 2
 INTERP_FAST, INTERP_FP32 = 0, 1
+(OPTION_INTERPOLATION, OPTION_BACKPROJECT_KERNEL, OPTION_FORWARD_KERNEL,
+ OPTION_FORWARD_ADDRESSING) = 0, 1, 2, 3
+KERNEL_AUTO, KERNEL_LDS, KERNEL_GATHER, KERNEL_RAY, KERNEL_POINTER = 0, 1, 2, 3, 4
 RAM_LAK, SHEPP_LOGAN, COSINE = 0, 1, 2
 CONE, PARALLEL = 0, 1
 PROJECTIONS, SINOGRAMS = 0, 1
@@ -57,6 +60,8 @@ SIGNATURES = {
     "ctr_workspace_create": [_v],
     "ctr_workspace_destroy": [_v],
     "ctr_workspace_reserve": [_v, _i, _i],
+    "ctr_workspace_set_option": [_v, _i, _i],
+    "ctr_workspace_get_option": [_v, _i],
     "ctr_fft_radix2": [_v, _v, _i, _i],
     "ctr_inverse_fft_radix2": [_v, _v, _i, _i],
     "ctr_fft_shift": [_v, _v, _v, _i, _i, _i],
@@ -74,8 +79,8 @@ SIGNATURES = {
     "ctr_stage_projections": [_v, _v, _v, _i, _i, _i, _i, _v, _i, _v],
     "ctr_backproject": [_v, _v, _i, _v, _i, _v, _i, _i, _i, _d, _d, _d, _d, _d,
                         _d, _v, _i, _i, _i, _f, _i, _i],
-    "ctr_forward_project": [_v, _i, _v, _i, _i, _i, _d, _d, _d, _d, _d, _d, _v,
-                            _i, _i, _i, _i, _d],
+    "ctr_forward_project": [_v, _v, _i, _v, _i, _i, _i, _d, _d, _d, _d, _d, _d,
+                            _v, _i, _i, _i, _i, _d],
     "ctr_fdk_weight": [_v, _v, _i, _i, _i, _i, _d, _d, _d],
     "ctr_log_normalize": [_v, _v, _sz, _f, _f],
     "ctr_clip_negative": [_v, _v, _sz],
@@ -87,8 +92,6 @@ SIGNATURES = {
     "ctr_min_max": [_v, _v, _v, _sz, _v],
     "ctr_normalize_range": [_v, _v, _sz, _v],
     "ctr_slices_unorm16": [_v, _v, _i, _i, _i, _i, _i, _v],
-    "ctr_set_interpolation": [_i],
-    "ctr_get_interpolation": [],
 }
 _RESTYPE = {"ctr_last_error": ctypes.c_char_p}
 
@@ -173,6 +176,22 @@ class Workspace:
 
     def reserve(self, n_views, max_fft_length=0):
         check(lib().ctr_workspace_reserve(self.handle, n_views, max_fft_length))
+
+    def set_option(self, option, value):
+        """ctr_workspace_set_option; returns the previous value."""
+        rc = lib().ctr_workspace_set_option(self.handle, int(option), int(value))
+        check(rc if rc < 0 else CTR_OK)
+        return rc
+
+    def get_option(self, option):
+        rc = lib().ctr_workspace_get_option(self.handle, int(option))
+        check(rc if rc < 0 else CTR_OK)
+        return rc
+
+    def set_interpolation(self, mode):
+        """INTERP_FP32 or INTERP_FAST for the 16-bit staged forms of every
+        backprojection given this workspace; returns the previous mode."""
+        return self.set_option(OPTION_INTERPOLATION, mode)
 
     def close(self):
         if self.handle:
@@ -338,17 +357,12 @@ def backproject(staged, geom, dtype=F32, view_scale=None, scale=1.0, volume=None
     return volume
 
 
-def set_interpolation(mode):
-    """Process-wide interpolation arithmetic of the 16-bit staged types:
-    INTERP_FAST or INTERP_FP32 (ctr_set_interpolation); returns the previous
-    mode."""
-    rc = lib().ctr_set_interpolation(int(mode))
+def default_option(option):
+    """Option value of the current device's default workspace (fixed at
+    library load from the environment)."""
+    rc = lib().ctr_workspace_get_option(None, int(option))
     check(rc if rc < 0 else CTR_OK)
     return rc
-
-
-def get_interpolation():
-    return lib().ctr_get_interpolation()
 
 
 def backproject_packed(items, dtype=F32, ws=None, stream=None):
@@ -370,7 +384,8 @@ def backproject_packed(items, dtype=F32, ws=None, stream=None):
     return [it[4] for it in items]
 
 
-def forward_project(volume, geom, layout=PROJECTIONS, step=1.0, out=None, stream=None):
+def forward_project(volume, geom, layout=PROJECTIONS, step=1.0, out=None, ws=None,
+                    stream=None):
     torch = _torch()
     _f32c(volume)
     if out is None:
@@ -378,7 +393,8 @@ def forward_project(volume, geom, layout=PROJECTIONS, step=1.0, out=None, stream
                  else (geom.nv, geom.n_views, geom.nu))
         out = torch.empty(shape, dtype=torch.float32, device=volume.device)
     check(lib().ctr_forward_project(
-        stream_handle(stream), geom.kind, ptr(volume), geom.X, geom.Y, geom.Z,
+        stream_handle(stream), ws.handle if ws else None, geom.kind, ptr(volume), geom.X,
+        geom.Y, geom.Z,
         geom.source_distance, geom.detector_distance, geom.detector_width,
         geom.volume_width, geom.rotation_count, geom.rotation_offset, ptr(out),
         layout, geom.nu, geom.nv, geom.n_views, float(step)))

@@ -31,7 +31,8 @@ struct Error : std::runtime_error {
 // Checks a ctr_* status code; raises with ctr_last_error() text on failure.
 void check_status(int rc, const char *what);
 
-// Stream all classes enqueue on (hipStream_t as void*, NULL = default).
+// Stream the calling thread's classes enqueue on (hipStream_t as void*,
+// NULL = default); each thread keeps its own.
 void set_stream(void *stream);
 void *get_stream();
 

@@ -66,7 +66,7 @@ void FBP2D::project_forward(Texture2D &source_image, Texture2D &target_image,
                             float total_rotation_count)
 {
     glm::ivec2 img = source_image.get_size(), sino = target_image.get_size();
-    check_status(ctr_forward_project(get_stream(), CTR_GEOMETRY_PARALLEL, source_image.data(),
+    check_status(ctr_forward_project(get_stream(), nullptr, CTR_GEOMETRY_PARALLEL, source_image.data(),
                                      img.x, 1, img.y, 0.0, 0.0, img.x * std::sqrt(2.0),
                                      (double)img.x, total_rotation_count, 0.0,
                                      target_image.data(), CTR_LAYOUT_PROJECTIONS, sino.x, 1,
@@ -82,7 +82,7 @@ void FBP2D::project_forward_equiangular(Texture2D &source_image, Texture2D &targ
                                         float volume_width, float total_rotation_count)
 {
     glm::ivec2 img = source_image.get_size(), sino = target_image.get_size();
-    check_status(ctr_forward_project(get_stream(), CTR_GEOMETRY_CONE, source_image.data(), img.x,
+    check_status(ctr_forward_project(get_stream(), nullptr, CTR_GEOMETRY_CONE, source_image.data(), img.x,
                                      1, img.y, xray_source_distance, detector_panel_distance,
                                      detector_width, volume_width, total_rotation_count, 0.0,
                                      target_image.data(), CTR_LAYOUT_PROJECTIONS, sino.x, 1,

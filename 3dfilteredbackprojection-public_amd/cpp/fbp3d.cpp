@@ -128,7 +128,7 @@ void FBP3D::project_forward_parallel_to_projections(float full_rotation_count,
             "FBP3D tried to project_forward_parallel() but FBP3D.volume was nullptr");
     glm::ivec3 v = volume->get_size();
     generate_blank_projections(v.x, v.y, projection_count);
-    check_status(ctr_forward_project(get_stream(), CTR_GEOMETRY_PARALLEL, volume->data(), v.x,
+    check_status(ctr_forward_project(get_stream(), nullptr, CTR_GEOMETRY_PARALLEL, volume->data(), v.x,
                                      v.y, v.z, 0.0, 0.0, (double)v.x, (double)v.x,
                                      full_rotation_count, rotation_offset_radian,
                                      projections->data(), CTR_LAYOUT_PROJECTIONS, v.x, v.y,
@@ -144,7 +144,7 @@ void FBP3D::project_forward_parallel_to_sinogram(float full_rotation_count, int 
             "FBP3D tried to project_forward_parallel_to_sinogram() but FBP3D.volume was nullptr");
     glm::ivec3 v = volume->get_size();
     generate_blank_sinograms(v.x, v.y, projection_count);
-    check_status(ctr_forward_project(get_stream(), CTR_GEOMETRY_PARALLEL, volume->data(), v.x,
+    check_status(ctr_forward_project(get_stream(), nullptr, CTR_GEOMETRY_PARALLEL, volume->data(), v.x,
                                      v.y, v.z, 0.0, 0.0, (double)v.x, (double)v.x,
                                      full_rotation_count, rotation_offset_radian,
                                      sinograms->data(), CTR_LAYOUT_SINOGRAMS, v.x, v.y,
@@ -167,7 +167,7 @@ void FBP3D::project_forward_cone_to_projections(float xray_source_distance,
     int nu = detector_resolution_u > 0 ? detector_resolution_u : v.x;
     int nv = detector_resolution_v > 0 ? detector_resolution_v : v.y;
     generate_blank_projections(nu, nv, projection_count);
-    check_status(ctr_forward_project(get_stream(), CTR_GEOMETRY_CONE, volume->data(), v.x, v.y,
+    check_status(ctr_forward_project(get_stream(), nullptr, CTR_GEOMETRY_CONE, volume->data(), v.x, v.y,
                                      v.z, xray_source_distance, detector_panel_distance,
                                      detector_width, volume_width, full_rotation_count,
                                      rotation_offset_radian, projections->data(),
@@ -188,7 +188,7 @@ void FBP3D::project_forward_cone_to_sinogram(float xray_source_distance,
     int nu = detector_resolution_u > 0 ? detector_resolution_u : v.x;
     int nv = detector_resolution_v > 0 ? detector_resolution_v : v.y;
     generate_blank_sinograms(nu, nv, projection_count);
-    check_status(ctr_forward_project(get_stream(), CTR_GEOMETRY_CONE, volume->data(), v.x, v.y,
+    check_status(ctr_forward_project(get_stream(), nullptr, CTR_GEOMETRY_CONE, volume->data(), v.x, v.y,
                                      v.z, xray_source_distance, detector_panel_distance,
                                      detector_width, volume_width, full_rotation_count,
                                      rotation_offset_radian, sinograms->data(),

@@ -110,11 +110,11 @@ void FFFT::fft_shift(Texture2D &source_complex_texture, Texture2D &target_comple
                  "FFFT::fft_shift");
 }
 
+// The reference shifts into a new texture and blits it back (:113-119); the
+// C ABI shifts in place without a temporary.
 void FFFT::fft_shift(Texture2D &complex_texture, int shift_amount)
 {
-    std::shared_ptr<Texture2D> copy = complex_texture.create_texture_with_same_parameters();
-    fft_shift(complex_texture, *copy, shift_amount);
-    blit_texture_complex_to_complex(*copy, complex_texture);
+    fft_shift(complex_texture, complex_texture, shift_amount);
 }
 
 void FFFT::inverse_fft_shift(Texture2D &source_complex_texture, Texture2D &target_complex_texture)
@@ -136,9 +136,7 @@ void FFFT::inverse_fft_shift(Texture2D &source_complex_texture, Texture2D &targe
 
 void FFFT::inverse_fft_shift(Texture2D &complex_texture, int shift_amount)
 {
-    std::shared_ptr<Texture2D> copy = complex_texture.create_texture_with_same_parameters();
-    inverse_fft_shift(complex_texture, *copy, shift_amount);
-    blit_texture_complex_to_complex(*copy, complex_texture);
+    fft_shift(complex_texture, complex_texture, -shift_amount);
 }
 
 // FastFourierTransformGPU.cpp:238-244

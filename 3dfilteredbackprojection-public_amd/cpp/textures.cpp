@@ -9,7 +9,9 @@
 
 namespace ctrecon {
 
-static void *g_stream = nullptr;
+// Per thread, like the reference's one GL context per thread: each thread's
+// classes enqueue on the stream that thread set.
+static thread_local void *g_stream = nullptr;
 
 void set_stream(void *stream) { g_stream = stream; }
 void *get_stream() { return g_stream; }

@@ -866,7 +866,7 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
     const size_t foot_stride = (size_t)J.gx * J.gy;
     const char *vbase = reinterpret_cast<const char *>(J.staged);
     const size_t view_bytes = J.view_texels * 4;
-    const unsigned view_bytes32 = (unsigned)view_bytes;  // < 2^24, checked by the host
+    const unsigned view_bytes32 = (unsigned)view_bytes;  // < 2^31, checked by the host
     const unsigned lp4 = J.lds_pitch4;
 
     // accumulation scale: fp16 weights are relative to the largest view scale
@@ -1025,19 +1025,14 @@ __global__ __launch_bounds__(256, lds_min_blocks<DT>()) void backproject_lds_pac
     backproject_lds_tile<DT, TY>(J, bx, by, bz, lds);
 }
 
-// Kernel choice: the LDS-staged kernel for volumes at least 32 voxels along
-// the rotation axis, the direct-tap kernel otherwise.  CTR_BACKPROJECT_KERNEL
-// = "gather" or "lds" overrides (diagnostics and A/B timing).
-static int variant_override()
-{
-    const char *e = getenv("CTR_BACKPROJECT_KERNEL");
-    if (!e) return 0;
-    if (!strcmp(e, "gather")) return 1;
-    if (!strcmp(e, "lds")) return 2;
-    return 0;
-}
-
 constexpr int kTY = CTR_LDS_TY;  // LDS kernel: voxels per thread along y
+
+// Byte offsets inside one staged view are 32-bit: the row DMA's buffer
+// resource (num_records a positive int) and the direct taps'
+// mad24(row, pitch) + base, whose operands (row < 2^21, pitch < 2^24 bytes
+// with nu < 2^20) fit 24 bits and whose 32-bit sum is exact modulo 2^32.
+// A 2048^2 panel is 2080 x 2051 staged texels (17 MB).
+constexpr size_t kMaxViewBytes = (size_t)1 << 31;
 
 static BpJob make_job(const void *staged, const StagedLayout &L, int nu, int nv, GeomPtrs geom,
                       const float *view_scale, int n_views, float *vol, int X, int Y, int Z,
@@ -1069,10 +1064,13 @@ static BpJob make_job(const void *staged, const StagedLayout &L, int nu, int nv,
     return J;
 }
 
-static bool use_lds_kernel(int Y)
+// Kernel choice: the LDS-staged kernel for volumes at least 32 voxels along
+// the rotation axis, the direct-tap kernel otherwise; the workspace's
+// CTR_OPTION_BACKPROJECT_KERNEL overrides (diagnostics and A/B timing).
+static bool use_lds_kernel(const ctr_workspace *ws, int Y)
 {
-    const int ov = variant_override();
-    return ov == 2 || (ov == 0 && Y >= 32);
+    const int ov = workspace_option(ws, CTR_OPTION_BACKPROJECT_KERNEL);
+    return ov == CTR_KERNEL_LDS || (ov == CTR_KERNEL_AUTO && Y >= 32);
 }
 
 // Fills the footprint table of job J at `dst` (gx * gy * n_views float4).
@@ -1118,36 +1116,21 @@ static void launch_lds_packed(hipStream_t s, BpBatch B, unsigned blocks)
     backproject_lds_packed_kernel<DT, kTY><<<blocks, 256, 0, s>>>(B);
 }
 
-// Interpolation arithmetic of the 16-bit formats, process-wide: CTR_INTERP_FP32
-// or CTR_INTERP_FAST (the kF16Packed / kBF16Dot variants).  The initial mode
-// is CTR_INTERP_FAST when the environment sets CTR_INTERPOLATION=fast.
-static std::atomic<int> g_interp{-1};
-
-static int interp_mode()
+// LDS tile variant of a staged format under the workspace's interpolation
+// arithmetic: the plain format (FP32 interpolation) or the kF16Packed /
+// kBF16Dot variants (CTR_INTERP_FAST).
+static int lds_variant(const ctr_workspace *ws, int staged_dtype, bool f16_weights_ok)
 {
-    int m = g_interp.load(std::memory_order_relaxed);
-    if (m < 0) {
-        const char *e = getenv("CTR_INTERPOLATION");
-        const int init = e && !strcmp(e, "fast") ? CTR_INTERP_FAST : CTR_INTERP_FP32;
-        g_interp.compare_exchange_strong(m, init);
-        m = g_interp.load(std::memory_order_relaxed);
-    }
-    return m;
-}
-
-// LDS tile variant of a staged format under the interpolation mode.
-static int lds_variant(int staged_dtype, bool f16_weights_ok)
-{
-    if (interp_mode() == CTR_INTERP_FP32) return staged_dtype;
+    if (workspace_option(ws, CTR_OPTION_INTERPOLATION) == CTR_INTERP_FP32) return staged_dtype;
     if (staged_dtype == CTR_F16) return f16_weights_ok ? kF16Packed : CTR_F16;
     if (staged_dtype == CTR_BF16) return kBF16Dot;
     return staged_dtype;
 }
 
 template <int DT>
-static int launch_bp(hipStream_t s, const BpJob &J, int variant)
+static int launch_bp(hipStream_t s, const BpJob &J, bool lds, int variant)
 {
-    if (use_lds_kernel(J.Y)) {
+    if (lds) {
         const long long blocks = (long long)J.gx * J.gy * ((J.Y + kTY - 1) / kTY);
         CTR_REQUIRE(blocks < (1LL << 31), CTR_ERR_INVALID_ARGUMENT, "volume too large");
         if (variant == kF16Packed)
@@ -1202,9 +1185,9 @@ int ctr_backproject(void *stream, ctr_workspace *ws, int geometry, const void *s
                             volume_width, X, Y, Z, nu, nv, n_views);
     if (rc) return rc;
     StagedLayout L = staged_layout(nu, nv, n_views, staged_dtype);
-    CTR_REQUIRE(L.view * 4 < ((size_t)1 << 24), CTR_ERR_INVALID_ARGUMENT,
-                "staged view of %zu texels exceeds the 2^22-texel limit of the "
-                "backprojector's in-view offsets", L.view);
+    CTR_REQUIRE(L.view * 4 < kMaxViewBytes, CTR_ERR_INVALID_ARGUMENT,
+                "staged view of %zu bytes exceeds the backprojector's 32-bit in-view "
+                "offsets (%zu bytes)", L.view * 4, kMaxViewBytes);
     ctr_workspace *w = nullptr;
     if ((rc = workspace_resolve(ws, &w))) return rc;
     hipStream_t s = as_stream(stream);
@@ -1218,31 +1201,20 @@ int ctr_backproject(void *stream, ctr_workspace *ws, int geometry, const void *s
         return rc;
     BpJob J = make_job(staged, L, nu, nv, g, view_scale, n_views, volume, X, Y, Z, scale,
                        accumulate, clip_negative);
-    if (use_lds_kernel(Y)) {
+    const bool lds = use_lds_kernel(w, Y);
+    if (lds) {
         void *f = nullptr;
         if ((rc = workspace_get(w, SLOT_FOOTPRINT, footprint_bytes(X, Z, n_views), &f))) return rc;
         if ((rc = build_footprints(s, J, (float4 *)f))) return rc;
     }
     const int variant =
-        lds_variant(staged_dtype, f16_weights_fit(J, geometry, source_distance, volume_width));
+        lds_variant(w, staged_dtype, f16_weights_fit(J, geometry, source_distance, volume_width));
     switch (staged_dtype) {
-    case CTR_F32: return launch_bp<CTR_F32>(s, J, variant);
-    case CTR_F16: return launch_bp<CTR_F16>(s, J, variant);
-    default: return launch_bp<CTR_BF16>(s, J, variant);
+    case CTR_F32: return launch_bp<CTR_F32>(s, J, lds, variant);
+    case CTR_F16: return launch_bp<CTR_F16>(s, J, lds, variant);
+    default: return launch_bp<CTR_BF16>(s, J, lds, variant);
     }
 }
-
-int ctr_set_interpolation(int mode)
-{
-    clear_error();
-    CTR_REQUIRE(mode == CTR_INTERP_FAST || mode == CTR_INTERP_FP32, CTR_ERR_INVALID_ARGUMENT,
-                "ctr_set_interpolation: unknown mode %d", mode);
-    const int prev = interp_mode();
-    g_interp.store(mode);
-    return prev;
-}
-
-int ctr_get_interpolation(void) { return interp_mode(); }
 
 int ctr_backproject_packed(void *stream, ctr_workspace *ws, int staged_dtype,
                            const ctr_backproject_job *jobs, int count)
@@ -1267,8 +1239,9 @@ int ctr_backproject_packed(void *stream, ctr_workspace *ws, int staged_dtype,
                                 j.n_views);
         if (rc) return rc;
         StagedLayout L = staged_layout(j.nu, j.nv, j.n_views, staged_dtype);
-        CTR_REQUIRE(L.view * 4 < ((size_t)1 << 24), CTR_ERR_INVALID_ARGUMENT,
-                    "job %d: staged view too large", q);
+        CTR_REQUIRE(L.view * 4 < kMaxViewBytes, CTR_ERR_INVALID_ARGUMENT,
+                    "job %d: staged view of %zu bytes exceeds the backprojector's 32-bit "
+                    "in-view offsets", q, L.view * 4);
         total_views += (size_t)j.n_views;
     }
     ctr_workspace *w = nullptr;
@@ -1304,7 +1277,7 @@ int ctr_backproject_packed(void *stream, ctr_workspace *ws, int staged_dtype,
         fp += footprint_bytes(j.X, j.Z, j.n_views);
     }
     CTR_REQUIRE(blocks < (1LL << 31), CTR_ERR_INVALID_ARGUMENT, "packed launch too large");
-    switch (lds_variant(staged_dtype, packed)) {
+    switch (lds_variant(w, staged_dtype, packed)) {
     case CTR_F32: launch_lds_packed<CTR_F32>(s, B, (unsigned)blocks); break;
     case CTR_F16: launch_lds_packed<CTR_F16>(s, B, (unsigned)blocks); break;
     case CTR_BF16: launch_lds_packed<CTR_BF16>(s, B, (unsigned)blocks); break;

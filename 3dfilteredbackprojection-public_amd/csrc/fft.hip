@@ -1215,6 +1215,37 @@ __global__ void shift_rows_kernel(const float2 *src, float2 *dst, int n, int row
     dst[idx] = src[row * n + j];
 }
 
+// In-place shift of rows of n <= MAXN complex values: one workgroup per row
+// holds the row in LDS, so the shift needs no second buffer.
+template <int MAXN>
+__global__ __launch_bounds__(256) void shift_rows_lds_kernel(float2 *data, int n, int shift)
+{
+    __shared__ float2 L[MAXN];
+    float2 *row = data + (size_t)blockIdx.x * n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) L[i] = row[i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        int j = i + shift;  // 0 <= shift < n
+        if (j >= n) j -= n;
+        row[j] = L[i];
+    }
+}
+
+// Reverses [a, a + len) of every row in place, one thread per swapped pair:
+// three reversals rotate a row longer than the LDS variant holds.
+__global__ void reverse_segment_kernel(float2 *data, int n, long long rows, int a, int len)
+{
+    const long long half = len / 2;
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= half * rows) return;
+    const long long r = idx / half;
+    const int p = (int)(idx - r * half);
+    float2 *row = data + (size_t)r * n + a;
+    const float2 t = row[p];
+    row[p] = row[len - 1 - p];
+    row[len - 1 - p] = t;
+}
+
 __global__ void blit_kernel(int mode, const float *src, int sw, float *dst, int dw,
                             int w, int h)
 {
@@ -1332,14 +1363,30 @@ int ctr_fft_shift(void *stream, const float *src, float *dst, int n, int rows, i
     if (total == 0) return CTR_OK;
     hipStream_t s = as_stream(stream);
     if (src == dst) {
-        // in-place overload (FastFourierTransformGPU.cpp:113-119): shift into
-        // a temporary, then copy back
-        float2 *tmp = nullptr;
-        CTR_TRY_HIP(hipMallocAsync((void **)&tmp, sizeof(float2) * total, s));
-        shift_rows_kernel<<<grid1d(total, 256), 256, 0, s>>>((const float2 *)src, tmp, n, rows, shift);
+        // in-place overload (FastFourierTransformGPU.cpp:113-119), with no
+        // temporary: a row of up to 8192 values is rotated through LDS, a
+        // longer one by three in-place reversals (all, then [0, sh), then
+        // [sh, n))
+        int sh = shift % n;
+        if (sh < 0) sh += n;
+        if (sh == 0) return CTR_OK;
+        float2 *d = (float2 *)dst;
+        CTR_REQUIRE(rows < (1 << 30), CTR_ERR_INVALID_ARGUMENT, "ctr_fft_shift: %d rows", rows);
+        if (n <= 1024) {
+            shift_rows_lds_kernel<1024><<<rows, 256, 0, s>>>(d, n, sh);
+        } else if (n <= kMaxN) {
+            shift_rows_lds_kernel<kMaxN><<<rows, 256, 0, s>>>(d, n, sh);
+        } else {
+            const int seg[3][2] = {{0, n}, {0, sh}, {sh, n - sh}};
+            for (const auto &g : seg) {
+                const long long pairs = (long long)(g[1] / 2) * rows;
+                if (pairs == 0) continue;
+                reverse_segment_kernel<<<grid1d(pairs, 256), 256, 0, s>>>(d, n, rows, g[0], g[1]);
+                CTR_CHECK_LAUNCH();
+            }
+            return CTR_OK;
+        }
         CTR_CHECK_LAUNCH();
-        CTR_TRY_HIP(hipMemcpyAsync(dst, tmp, sizeof(float2) * total, hipMemcpyDeviceToDevice, s));
-        CTR_TRY_HIP(hipFreeAsync(tmp, s));
         return CTR_OK;
     }
     shift_rows_kernel<<<grid1d(total, 256), 256, 0, s>>>((const float2 *)src, (float2 *)dst, n,

@@ -719,8 +719,8 @@ using namespace ctr;
 
 extern "C" {
 
-int ctr_forward_project(void *stream, int geometry, const float *volume, int X, int Y, int Z,
-                        double source_distance, double detector_distance,
+int ctr_forward_project(void *stream, ctr_workspace *ws, int geometry, const float *volume,
+                        int X, int Y, int Z, double source_distance, double detector_distance,
                         double detector_width, double volume_width, double rotation_count,
                         double rotation_offset, float *out, int layout, int nu, int nv,
                         int n_views, double step)
@@ -734,6 +734,8 @@ int ctr_forward_project(void *stream, int geometry, const float *volume, int X, 
     int rc = check_geometry(geometry, source_distance, detector_distance, detector_width,
                             volume_width, X, Y, Z, nu, nv, n_views);
     if (rc) return rc;
+    ctr_workspace *w = nullptr;
+    if ((rc = workspace_resolve(ws, &w))) return rc;
     ForwardJob J;
     J.geometry = geometry;
     J.vol = volume;
@@ -757,19 +759,17 @@ int ctr_forward_project(void *stream, int geometry, const float *volume, int X, 
     CTR_REQUIRE(blocks < (1LL << 31), CTR_ERR_INVALID_ARGUMENT, "forward projection too large");
     const dim3 grid((unsigned)blocks);
     // interior cells through a buffer resource: byte offsets below 4 GiB and
-    // the 24-bit products of the cell offset
-    // (CTR_FORWARD_ADDRESSING=pointer forces the 64-bit path: diagnostics
-    // and tests)
+    // the 24-bit products of the cell offset (the workspace option
+    // CTR_OPTION_FORWARD_ADDRESSING = CTR_KERNEL_POINTER forces the 64-bit
+    // path: diagnostics and tests)
     const unsigned long long bytes = 4ull * (unsigned long long)X * Y * Z;
-    const char *force = getenv("CTR_FORWARD_ADDRESSING");
     const bool buf = bytes <= 0xfffffff0ull && (unsigned long long)X * Y < (1ull << 24) &&
                      X < (1 << 24) && Y < (1 << 24) && Z < (1 << 24) &&
-                     !(force && !strcmp(force, "pointer"));
+                     workspace_option(w, CTR_OPTION_FORWARD_ADDRESSING) != CTR_KERNEL_POINTER;
     // the slab-staged kernel by default, its boxes filled through the same
-    // buffer resource or 64-bit addresses (CTR_FORWARD_KERNEL=ray forces the
-    // ray kernel: diagnostics and tests)
-    const char *kern = getenv("CTR_FORWARD_KERNEL");
-    const bool slab = !(kern && !strcmp(kern, "ray"));
+    // buffer resource or 64-bit addresses (CTR_OPTION_FORWARD_KERNEL =
+    // CTR_KERNEL_RAY forces the ray kernel: diagnostics and tests)
+    const bool slab = workspace_option(w, CTR_OPTION_FORWARD_KERNEL) != CTR_KERNEL_RAY;
     const long long tiles = (long long)((nu + kSlabTile - 1) / kSlabTile) *
                             ((nv + kSlabTile - 1) / kSlabTile);
     const long long slab_blocks =

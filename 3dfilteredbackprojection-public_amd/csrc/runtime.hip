@@ -1,6 +1,8 @@
 // runtime.hip -- errors, workspaces and staged-layout bookkeeping.
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -10,10 +12,55 @@ struct ctr_workspace {
     int device;
     void *buf[ctr::SLOT_COUNT];
     size_t cap[ctr::SLOT_COUNT];
+    std::atomic<int> option[CTR_OPTION_COUNT];
     std::mutex lock;
 };
 
 namespace ctr {
+
+// Initial option values of every workspace, from the environment as it is
+// when the library is loaded (read once, here; never again per call).
+struct OptionDefaults {
+    int value[CTR_OPTION_COUNT];
+    OptionDefaults()
+    {
+        value[CTR_OPTION_INTERPOLATION] = env_is("CTR_INTERPOLATION", "fast")
+                                              ? CTR_INTERP_FAST
+                                              : CTR_INTERP_FP32;
+        value[CTR_OPTION_BACKPROJECT_KERNEL] =
+            env_is("CTR_BACKPROJECT_KERNEL", "lds")      ? CTR_KERNEL_LDS
+            : env_is("CTR_BACKPROJECT_KERNEL", "gather") ? CTR_KERNEL_GATHER
+                                                         : CTR_KERNEL_AUTO;
+        value[CTR_OPTION_FORWARD_KERNEL] =
+            env_is("CTR_FORWARD_KERNEL", "ray") ? CTR_KERNEL_RAY : CTR_KERNEL_AUTO;
+        value[CTR_OPTION_FORWARD_ADDRESSING] =
+            env_is("CTR_FORWARD_ADDRESSING", "pointer") ? CTR_KERNEL_POINTER : CTR_KERNEL_AUTO;
+    }
+    static bool env_is(const char *name, const char *want)
+    {
+        const char *e = getenv(name);
+        return e && !strcmp(e, want);
+    }
+};
+static const OptionDefaults g_option_defaults;
+
+static bool option_value_ok(int option, int value)
+{
+    switch (option) {
+    case CTR_OPTION_INTERPOLATION: return value == CTR_INTERP_FP32 || value == CTR_INTERP_FAST;
+    case CTR_OPTION_BACKPROJECT_KERNEL:
+        return value == CTR_KERNEL_AUTO || value == CTR_KERNEL_LDS || value == CTR_KERNEL_GATHER;
+    case CTR_OPTION_FORWARD_KERNEL: return value == CTR_KERNEL_AUTO || value == CTR_KERNEL_RAY;
+    case CTR_OPTION_FORWARD_ADDRESSING:
+        return value == CTR_KERNEL_AUTO || value == CTR_KERNEL_POINTER;
+    default: return false;
+    }
+}
+
+int workspace_option(const ctr_workspace *ws, int option)
+{
+    return ws->option[option].load(std::memory_order_relaxed);
+}
 
 static thread_local char g_error[512] = {0};
 
@@ -107,14 +154,41 @@ int ctr_synchronize(void)
 int ctr_workspace_create(ctr_workspace **out)
 {
     CTR_REQUIRE(out, CTR_ERR_NULL_POINTER, "ctr_workspace_create: out is NULL");
+    int device = 0;
+    CTR_TRY_HIP(hipGetDevice(&device));
     ctr_workspace *ws = new ctr_workspace();
-    CTR_TRY_HIP(hipGetDevice(&ws->device));
+    ws->device = device;
     for (int i = 0; i < ctr::SLOT_COUNT; i++) {
         ws->buf[i] = nullptr;
         ws->cap[i] = 0;
     }
+    for (int o = 0; o < CTR_OPTION_COUNT; o++)
+        ws->option[o].store(ctr::g_option_defaults.value[o]);
     *out = ws;
     return CTR_OK;
+}
+
+int ctr_workspace_set_option(ctr_workspace *ws, int option, int value)
+{
+    ctr::clear_error();
+    CTR_REQUIRE(ws, CTR_ERR_INVALID_ARGUMENT,
+                "ctr_workspace_set_option: the default workspace (NULL) is shared by every "
+                "caller on the device; its options are fixed at library load");
+    CTR_REQUIRE(option >= 0 && option < CTR_OPTION_COUNT, CTR_ERR_INVALID_ARGUMENT,
+                "ctr_workspace_set_option: unknown option %d", option);
+    CTR_REQUIRE(ctr::option_value_ok(option, value), CTR_ERR_INVALID_ARGUMENT,
+                "ctr_workspace_set_option: value %d is not valid for option %d", value, option);
+    return ws->option[option].exchange(value);
+}
+
+int ctr_workspace_get_option(ctr_workspace *ws, int option)
+{
+    ctr::clear_error();
+    CTR_REQUIRE(option >= 0 && option < CTR_OPTION_COUNT, CTR_ERR_INVALID_ARGUMENT,
+                "ctr_workspace_get_option: unknown option %d", option);
+    // the default workspaces keep the load-time values: no device needed
+    if (!ws) return ctr::g_option_defaults.value[option];
+    return ctr::workspace_option(ws, option);
 }
 
 int ctr_workspace_destroy(ctr_workspace *ws)

@@ -8,7 +8,8 @@ Poisson noise at I0 = 1e5 and log.  The headline value is the FP16-stored
 form (filtered projections in FP16 with a per-view scale, FP32 interpolation
 and accumulation); the FP32-stored and BF16-stored forms are timed beside it,
 and the 16-bit forms once more with the 16-bit interpolation arithmetic
-(ctr_set_interpolation CTR_INTERP_FAST) as a secondary figure.
+(workspace option CTR_OPTION_INTERPOLATION = CTR_INTERP_FAST) as a secondary
+figure.
 
 Prints ONE JSON line.  Roofline for the dominant kernel (backprojection):
 algorithmic bytes per voxel-view update = 4 taps x element size + 8/N bytes
@@ -230,7 +231,7 @@ def main():
     log(f"workload {WORKLOAD}: {g.X}x{g.Y}x{g.Z} from {g.n_views} views of "
         f"{g.nu}x{g.nv}, FFT {FFT_LENGTH}")
     proj = make_inputs(ctr, g, dev)
-    ctr.set_interpolation(ctr.INTERP_FP32)
+    ws.set_interpolation(ctr.INTERP_FP32)
 
     log("timing FP32-stored form")
     f32_ms, f32_bp, vol32 = time_fdk(ctr, g, proj, ctr.F32, args.steps, args.warmup, ws, scale)
@@ -246,7 +247,7 @@ def main():
     rel_rmse_bf = float((volb.double() - ref32.double()).norm() / ref32.double().norm())
     del volb
     log("timing FP16 / BF16 with the 16-bit interpolation arithmetic")
-    ctr.set_interpolation(ctr.INTERP_FAST)
+    ws.set_interpolation(ctr.INTERP_FAST)
     fast = {}
     for name, dt in (("fp16", ctr.F16), ("bf16", ctr.BF16)):
         t_ms, t_bp, v = time_fdk(ctr, g, proj, dt, max(3, args.steps // 2), 1, ws, scale)
@@ -257,7 +258,7 @@ def main():
             "rel_rmse_vs_fp32": float((v.double() - ref32.double()).norm()
                                       / ref32.double().norm())}
         del v
-    ctr.set_interpolation(ctr.INTERP_FP32)
+    ws.set_interpolation(ctr.INTERP_FP32)
 
     torch.cuda.synchronize()
     ms = statistics.mean(f16_ms)

@@ -50,9 +50,23 @@ extern "C" {
 #define CTR_F16 1
 #define CTR_BF16 2
 
-/* interpolation arithmetic of the 16-bit staged types (ctr_set_interpolation) */
+/* workspace options (ctr_workspace_set_option) */
+#define CTR_OPTION_INTERPOLATION 0      /* CTR_INTERP_FP32 (default) or CTR_INTERP_FAST */
+#define CTR_OPTION_BACKPROJECT_KERNEL 1 /* CTR_KERNEL_AUTO (default), _LDS or _GATHER */
+#define CTR_OPTION_FORWARD_KERNEL 2     /* CTR_KERNEL_AUTO (default: slab) or _RAY */
+#define CTR_OPTION_FORWARD_ADDRESSING 3 /* CTR_KERNEL_AUTO (default) or _POINTER */
+#define CTR_OPTION_COUNT 4
+
+/* interpolation arithmetic of the 16-bit staged types */
 #define CTR_INTERP_FAST 0
 #define CTR_INTERP_FP32 1
+
+/* kernel choices of the diagnostic options */
+#define CTR_KERNEL_AUTO 0
+#define CTR_KERNEL_LDS 1     /* backprojection: LDS-staged kernel at any size */
+#define CTR_KERNEL_GATHER 2  /* backprojection: direct-tap kernel */
+#define CTR_KERNEL_RAY 3     /* forward projection: ray kernel instead of slab */
+#define CTR_KERNEL_POINTER 4 /* forward projection: 64-bit addressing */
 
 /* FBP2D::FilterType (Source/FilteredBackProjection/FBP2D.h:10-14) */
 #define CTR_RAM_LAK 0
@@ -96,6 +110,24 @@ int ctr_synchronize(void);
 int ctr_workspace_create(ctr_workspace **out);
 int ctr_workspace_destroy(ctr_workspace *ws);
 int ctr_workspace_reserve(ctr_workspace *ws, int n_views, int max_fft_length);
+
+/* Per-workspace options: the interpolation arithmetic of the 16-bit staged
+ * forms (CTR_OPTION_INTERPOLATION) and diagnostic kernel choices.  Every
+ * entry point reads the options of the workspace it is given when it
+ * launches, so two threads driving two streams with two workspaces can use
+ * different settings at the same time; a HIP graph keeps the kernels it
+ * captured.  A new workspace -- and the per-device default workspace that
+ * NULL selects -- starts from the environment as read once when the library
+ * is loaded: CTR_INTERPOLATION=fast, CTR_BACKPROJECT_KERNEL=lds|gather,
+ * CTR_FORWARD_KERNEL=ray, CTR_FORWARD_ADDRESSING=pointer.  The default
+ * workspace is shared by every NULL caller on a device, so its options
+ * cannot be changed (CTR_ERR_INVALID_ARGUMENT for ws == NULL).
+ * ctr_workspace_set_option returns the previous value.  No reference
+ * counterpart (its sampler interpolated FP32 texels; one GL context). */
+int ctr_workspace_set_option(ctr_workspace *ws, int option, int value);
+/* The option's value for ws (NULL: the default workspaces' value, which is
+ * the one read from the environment at library load). */
+int ctr_workspace_get_option(ctr_workspace *ws, int option);
 
 /* ---- FFT family: FFFT (Source/FastFourierTransformGPU/) --------------------
  * Complex data are interleaved float pairs (RG32F texture), rows of n
@@ -207,8 +239,9 @@ int ctr_stage_projections(void *stream, ctr_workspace *ws, const float *proj,
  * backprojection).  view_scale may be NULL for CTR_F32.  Coordinates are set
  * up in FP64 per column and view and stepped in FP32 along y; accumulation
  * is FP32.  Interpolation is FP32 (16-bit pairs widened exactly) under
- * CTR_INTERP_FP32, the default mode of ctr_set_interpolation, and for every
- * format on volumes under 32 voxels along y.  Under CTR_INTERP_FAST the
+ * CTR_INTERP_FP32, the default CTR_OPTION_INTERPOLATION of the workspace,
+ * and for every format on volumes under 32 voxels along y.  Under
+ * CTR_INTERP_FAST the
  * 16-bit formats use their own arithmetic for the interpolation weights:
  * CTR_F16 pairs are interpolated in packed f16 (per-sample relative error
  * ~2^-11; about 4e-4 rel RMSE on white-noise projections, 4e-5 on the 512^3
@@ -248,20 +281,6 @@ typedef struct ctr_backproject_job {
 int ctr_backproject_packed(void *stream, ctr_workspace *ws, int staged_dtype,
                            const ctr_backproject_job *jobs, int count);
 
-/* Interpolation arithmetic of ctr_backproject and ctr_backproject_packed for
- * CTR_F16 / CTR_BF16 staged projections, for the whole process and every
- * later call: CTR_INTERP_FP32 (pairs widened to FP32, interpolated and
- * weighted in FP32) or CTR_INTERP_FAST (the 16-bit arithmetic described at
- * ctr_backproject: 10-15% faster, larger error).  The initial mode is
- * CTR_INTERP_FAST when the environment variable CTR_INTERPOLATION is "fast"
- * at the first call, else CTR_INTERP_FP32.  The mode picks the kernel at
- * each launch: a HIP graph captured before a change keeps replaying the
- * kernel it captured.  ctr_set_interpolation returns the previous mode, or
- * CTR_ERR_INVALID_ARGUMENT for an unknown one.  No reference counterpart
- * (the reference interpolates FP32 texels in the sampler). */
-int ctr_set_interpolation(int mode);
-int ctr_get_interpolation(void);
-
 /* ---- forward projection ------------------------------------------------------
  * Ray-driven projector, one ray per detector pixel centre, replacing
  *   FBP3D::project_forward_cone_to_projections   (FBP3D.cpp:269-299)
@@ -271,8 +290,9 @@ int ctr_get_interpolation(void);
  *   FBP2D::project_forward / project_forward_equiangular (FBP2D.cpp:54-94)
  * The ray is clipped to the volume box grown by one voxel per side; samples
  * at t_in + (k + 1/2) h, h = step * d, trilinear (zero outside), times h.
- * Output overwrites `out` in the given layout. */
-int ctr_forward_project(void *stream, int geometry, const float *volume,
+ * Output overwrites `out` in the given layout.  ws supplies the kernel
+ * options (CTR_OPTION_FORWARD_*; NULL: the default workspace). */
+int ctr_forward_project(void *stream, ctr_workspace *ws, int geometry, const float *volume,
                         int X, int Y, int Z, double source_distance,
                         double detector_distance, double detector_width,
                         double volume_width, double rotation_count,

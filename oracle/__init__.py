@@ -46,6 +46,8 @@ def lib():
         L.orc_fft_radix2.argtypes = [_p, _p, _i, _d]
         L.orc_inverse_fft_radix2.argtypes = [_p, _p, _i, _d]
         L.orc_fft_shift.argtypes = [_p, _p, _i, _i]
+        L.orc_fft_bit_reverse.argtypes = [_p, _p, _i]
+        L.orc_fft_single_stage.argtypes = [_p, _p, _p, _p, _i, _i, _d]
         L.orc_dft.argtypes = [_p, _p, _p, _p, _i, _i]
         L.orc_filter_gain.argtypes = [_i, _i, _i, _d]
         L.orc_filter_gain.restype = _d
@@ -144,6 +146,26 @@ def fft_shift(z, shift=None):
     re, im = _f64(z.real.copy()), _f64(z.imag.copy())
     lib().orc_fft_shift(_ptr(re), _ptr(im), n, int(shift))
     return re + 1j * im
+
+
+def fft_bit_reverse(z):
+    """Bit-reversal permutation by swaps (the first pass of the FFT)."""
+    z = np.asarray(z, dtype=np.complex128)
+    re, im = _f64(z.real.copy()), _f64(z.imag.copy())
+    if lib().orc_fft_bit_reverse(_ptr(re), _ptr(im), re.size) != 0:
+        raise ValueError("length is not a power of 2")
+    return re + 1j * im
+
+
+def fft_single_stage(z, step_index, pi=np.pi):
+    """One radix-2 stage of span 2 << step_index (read z, return the write)."""
+    z = np.asarray(z, dtype=np.complex128)
+    re, im = _f64(z.real.copy()), _f64(z.imag.copy())
+    wre, wim = np.empty_like(re), np.empty_like(im)
+    if lib().orc_fft_single_stage(_ptr(re), _ptr(im), _ptr(wre), _ptr(wim), re.size,
+                                  int(step_index), pi) != 0:
+        raise ValueError("bad length or stage")
+    return wre + 1j * wim
 
 
 def dft(z, sign=-1):

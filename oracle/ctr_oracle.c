@@ -101,6 +101,58 @@ int orc_fft_radix2(double *re, double *im, int n, double pi)
     return 0;
 }
 
+int orc_fft_bit_reverse(double *re, double *im, int n)
+{
+    /* The swap loop alone: FastFourierTransform.cpp:63-67, which
+     * FFFT::reverse_bit_swap runs on every texture row
+     * (FastFourierTransformGPU.cpp:238-244). */
+    if (n < 1) return -1;
+    int levels = floor_log2_i(n);
+    if ((1L << levels) != (long)n) return -1;
+    for (unsigned long i = 0; i < (unsigned long)n; i++) {
+        unsigned long j = reverse_bits_ul(i, levels);
+        if (j > i) {
+            double t = re[i]; re[i] = re[j]; re[j] = t;
+            t = im[i]; im[i] = im[j]; im[j] = t;
+        }
+    }
+    return 0;
+}
+
+int orc_fft_single_stage(const double *rre, const double *rim, double *wre,
+                         double *wim, int n, int step_index, double pi)
+{
+    /* One butterfly stage, one output per element:
+     * ParallelFastFourierTransform.cpp:262-277 (the CPU guide of
+     * FFFT::fft_single_step, FastFourierTransformGPU.cpp:246-256), with the
+     * n/2-entry table polar(1, -2 pi k / n) of :113-115. */
+    if (n < 2) return -1;
+    int levels = floor_log2_i(n);
+    if ((1L << levels) != (long)n || step_index < 0 || (2L << step_index) > (long)n)
+        return -1;
+    const int size = 2 << step_index, half = size / 2, tablestep = n / size;
+    for (int i = 0; i < n; i++) {
+        const int local = i - (i / size) * size;
+        const int k = (local % half) * tablestep;
+        const double phase = -2.0 * pi * (double)k / (double)n;
+        const double tr = cos(phase), ti = sin(phase);
+        if (local >= half) {
+            /* read[i - half] + (-1) * (read[i] * w) */
+            const double pr = rre[i] * tr - rim[i] * ti;
+            const double pi_ = rre[i] * ti + rim[i] * tr;
+            wre[i] = rre[i - half] + (-pr);
+            wim[i] = rim[i - half] + (-pi_);
+        } else {
+            /* read[i] + read[i + half] * w */
+            const double pr = rre[i + half] * tr - rim[i + half] * ti;
+            const double pi_ = rre[i + half] * ti + rim[i + half] * tr;
+            wre[i] = rre[i] + pr;
+            wim[i] = rim[i] + pi_;
+        }
+    }
+    return 0;
+}
+
 int orc_inverse_fft_radix2(double *re, double *im, int n, double pi)
 {
     /* FastFourierTransform.cpp:90-106: conj, FFT, conj, divide by n */

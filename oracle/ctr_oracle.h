@@ -50,6 +50,16 @@ double orc_reference_pi(void);
  * constant (M_PI or orc_reference_pi()).  Returns -1 if n is not a power of
  * two (FastFourierTransform.cpp:55-56), else 0. */
 int orc_fft_radix2(double *re, double *im, int n, double pi);
+/* The bit-reversal swap pass of the FFT on its own
+ * (FastFourierTransform.cpp:63-67; FFFT::reverse_bit_swap,
+ * FastFourierTransformGPU.cpp:238-244).  -1 unless n is a power of two. */
+int orc_fft_bit_reverse(double *re, double *im, int n);
+/* One radix-2 stage of span 2 << step_index from read into write, one
+ * output per element (ParallelFastFourierTransform.cpp:262-277;
+ * FFFT::fft_single_step, FastFourierTransformGPU.cpp:246-256).  Bit
+ * reversal followed by stages 0 .. log2(n) - 1 is the FFT. */
+int orc_fft_single_stage(const double *rre, const double *rim, double *wre,
+                         double *wim, int n, int step_index, double pi);
 /* conj -> FFT -> conj -> /n (FastFourierTransform.cpp:90-106). */
 int orc_inverse_fft_radix2(double *re, double *im, int n, double pi);
 /* dst[(i + shift) mod n] = src[i] (FastFourierTransform.cpp:108-125). */

@@ -33,15 +33,20 @@ def dev(ctr):
     return torch.device("cuda:0")
 
 
+def _wide(a):
+    """float64, or complex128 when the input is complex (so that |got - ref|
+    includes the imaginary parts)."""
+    a = np.asarray(a)
+    return a.astype(np.complex128 if np.iscomplexobj(a) else np.float64)
+
+
 def rel_max(got, ref):
-    got = np.asarray(got, dtype=np.float64)
-    ref = np.asarray(ref, dtype=np.float64)
+    got, ref = _wide(got), _wide(ref)
     return float(np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-300))
 
 
 def rel_rmse(got, ref):
-    got = np.asarray(got, dtype=np.float64)
-    ref = np.asarray(ref, dtype=np.float64)
+    got, ref = _wide(got), _wide(ref)
     return float(np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-300))
 
 

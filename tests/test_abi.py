@@ -60,8 +60,8 @@ def test_argument_errors_without_device(ctr):
                                10.0, 1.0, 0.0, fake, 8, 8, 8, 1.0, 0, 0) == -5
     assert lib.ctr_backproject(None, None, 5, fake, 0, None, 8, 8, 4, 100.0, 1.0, 10.0,
                                10.0, 1.0, 0.0, fake, 8, 8, 8, 1.0, 0, 0) == -1
-    assert lib.ctr_forward_project(None, 0, fake, 0, 8, 8, 100.0, 1.0, 10.0, 10.0, 1.0,
-                                   0.0, fake, 0, 8, 8, 4, 1.0) == -1
+    assert lib.ctr_forward_project(None, None, 0, fake, 0, 8, 8, 100.0, 1.0, 10.0, 10.0,
+                                   1.0, 0.0, fake, 0, 8, 8, 4, 1.0) == -1
     assert lib.ctr_render_ellipsoids(None, None, fake, 5000, fake, 4, 4, 4, 4.0) == -1
     assert lib.ctr_staged_layout(0, 4, 4, 0, None, None, None) == -1
 
@@ -80,41 +80,48 @@ def test_product_path_fails_loudly_without_library(ctr, monkeypatch):
         ctr.lib()
 
 
-def test_interpolation_mode_switch(ctr):
-    """ctr_set_interpolation: returns the previous mode, rejects unknown ones
-    and leaves the mode unchanged then; no device is touched."""
+def test_workspace_options_without_device(ctr):
+    """The default workspace (NULL) reports the load-time options and refuses
+    changes; unknown options and values are rejected; no device is touched."""
     lib = ctr.lib()
-    start = lib.ctr_get_interpolation()
-    assert start in (ctr.INTERP_FAST, ctr.INTERP_FP32)
-    try:
-        assert ctr.set_interpolation(ctr.INTERP_FP32) == start
-        assert ctr.get_interpolation() == ctr.INTERP_FP32
-        assert lib.ctr_set_interpolation(7) == -1
-        assert "unknown mode" in ctr.last_error()
-        assert ctr.get_interpolation() == ctr.INTERP_FP32
-        with pytest.raises(ctr.CtrError):
-            ctr.set_interpolation(-1)
-        assert ctr.set_interpolation(ctr.INTERP_FAST) == ctr.INTERP_FP32
-        assert ctr.get_interpolation() == ctr.INTERP_FAST
-    finally:
-        ctr.set_interpolation(start)
+    assert lib.ctr_workspace_get_option(None, ctr.OPTION_INTERPOLATION) in (
+        ctr.INTERP_FAST, ctr.INTERP_FP32)
+    assert lib.ctr_workspace_set_option(None, ctr.OPTION_INTERPOLATION,
+                                        ctr.INTERP_FAST) == -1
+    assert "default workspace" in ctr.last_error()
+    assert lib.ctr_workspace_get_option(None, 9) == -1
+    assert "unknown option" in ctr.last_error()
+    assert lib.ctr_workspace_get_option(None, -1) == -1
 
 
-@pytest.mark.parametrize("value,mode", [("fp32", 1), ("fast", 0), ("", 1)])
-def test_interpolation_mode_from_environment(ctr, value, mode):
-    code = ("import ctypes, sys; L = ctypes.CDLL(sys.argv[1]); "
-            "print(L.ctr_get_interpolation())")
-    env = dict(os.environ, CTR_INTERPOLATION=value)
-    out = subprocess.run(["python", "-c", code, ctr.LIB_PATH], env=env,
-                         capture_output=True, text=True, check=True).stdout
-    assert int(out.strip()) == mode
+@pytest.mark.parametrize("env,option,value", [
+    ({"CTR_INTERPOLATION": "fp32"}, 0, 1), ({"CTR_INTERPOLATION": "fast"}, 0, 0),
+    ({"CTR_INTERPOLATION": ""}, 0, 1),
+    ({"CTR_BACKPROJECT_KERNEL": "lds"}, 1, 1), ({"CTR_BACKPROJECT_KERNEL": "gather"}, 1, 2),
+    ({"CTR_FORWARD_KERNEL": "ray"}, 2, 3), ({"CTR_FORWARD_ADDRESSING": "pointer"}, 3, 4),
+    ({}, 1, 0), ({}, 2, 0), ({}, 3, 0)])
+def test_options_read_from_environment_at_load(ctr, env, option, value):
+    """The environment is read once, when the library is loaded: changing it
+    afterwards changes nothing."""
+    code = ("import ctypes, os, sys; L = ctypes.CDLL(sys.argv[1]); "
+            "a = L.ctr_workspace_get_option(None, int(sys.argv[2])); "
+            "os.environ['CTR_INTERPOLATION'] = 'fast'; "
+            "os.environ['CTR_BACKPROJECT_KERNEL'] = 'gather'; "
+            "os.environ['CTR_FORWARD_KERNEL'] = 'ray'; "
+            "os.environ['CTR_FORWARD_ADDRESSING'] = 'pointer'; "
+            "b = L.ctr_workspace_get_option(None, int(sys.argv[2])); print(a, b)")
+    base = {k: v for k, v in os.environ.items() if not k.startswith("CTR_")}
+    out = subprocess.run(["python", "-c", code, ctr.LIB_PATH, str(option)],
+                         env=dict(base, **env), capture_output=True, text=True,
+                         check=True).stdout.split()
+    assert [int(x) for x in out] == [value, value]
 
 
 def test_size_limits_without_device(ctr):
     """Degenerate and over-limit sizes are rejected on the host, before any
     device call: empty view sets, sizes of 2^20 and more, a staged view past
-    the backprojector's 2^22-texel offsets, packed counts outside 1..16 and a
-    non-positive forward step."""
+    the backprojector's 32-bit in-view offsets (2^31 bytes), packed counts
+    outside 1..16 and a non-positive forward step."""
     lib = ctr.lib()
     fake = ctypes.c_void_p(64)
     bp = lambda nu, nv, n, X, dt=0, vs=None: lib.ctr_backproject(
@@ -124,15 +131,15 @@ def test_size_limits_without_device(ctr):
     assert "sizes must be positive" in ctr.last_error()
     assert bp(64, 64, 8, 1 << 20) == -1
     assert "below 2^20" in ctr.last_error()
-    # 2100 x 2100 detector: a staged view of 2112 x 2103 texels > 2^22
-    assert bp(2100, 2100, 8, 32) == -1
-    assert "2^22" in ctr.last_error()
-    assert bp(2100, 2100, 8, 32, dt=1, vs=fake) == -1
+    # 24000^2 detector: a staged view of 24032 x 24003 texels, 2.3e9 bytes
+    assert bp(24000, 24000, 8, 32) == -1
+    assert "32-bit in-view offsets" in ctr.last_error()
+    assert bp(24000, 24000, 8, 32, dt=1, vs=fake) == -1
     jobs = (ctr.BackprojectJob * 17)()
     assert lib.ctr_backproject_packed(None, None, 0, jobs, 0) == -1
     assert lib.ctr_backproject_packed(None, None, 0, jobs, 17) == -1
     assert lib.ctr_backproject_packed(None, None, 0, None, 1) == -5
     assert lib.ctr_backproject_packed(None, None, 7, jobs, 1) == -1
-    assert lib.ctr_forward_project(None, 0, fake, 8, 8, 8, 100.0, 1.0, 10.0, 10.0, 1.0,
-                                   0.0, fake, 0, 8, 8, 4, 0.0) == -1
+    assert lib.ctr_forward_project(None, None, 0, fake, 8, 8, 8, 100.0, 1.0, 10.0, 10.0,
+                                   1.0, 0.0, fake, 0, 8, 8, 4, 0.0) == -1
     assert "step must be positive" in ctr.last_error()

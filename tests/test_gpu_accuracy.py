@@ -51,16 +51,17 @@ def forms(ctr):
             ("fp16/fast", ctr.F16), ("bf16/fast", ctr.BF16))
 
 
+_WS = {}
+
+
 def mode(ctr, name):
-    """Sets the interpolation mode of a form (restored after each test)."""
-    ctr.set_interpolation(ctr.INTERP_FAST if name.endswith("fast") else ctr.INTERP_FP32)
-
-
-@pytest.fixture(autouse=True)
-def _restore_interpolation(ctr):
-    start = ctr.get_interpolation()
-    yield
-    ctr.set_interpolation(start)
+    """Workspace whose interpolation arithmetic is the form's (FP32 for the
+    plain forms, the 16-bit arithmetic for the /fast forms)."""
+    fast = name.endswith("fast")
+    if fast not in _WS:
+        _WS[fast] = ctr.Workspace()
+        _WS[fast].set_interpolation(ctr.INTERP_FAST if fast else ctr.INTERP_FP32)
+    return _WS[fast]
 
 
 def _t(dev, a):
@@ -78,9 +79,10 @@ def test_accuracy_par2d_256(ctr, orc, dev):
     ref = orc.backproject_parallel(q, g.detector_width, g.volume_width, g.rotation_count,
                                    0.0, scale, (g.X, 1, g.Z))
     for name, dt in forms(ctr):
-        mode(ctr, name)
-        staged, vs = ctr.filter_projections(_t(dev, sino), g, ctr.RAM_LAK, nfft, dt, fdk=False)
-        img = ctr.backproject(staged, g, dt, view_scale=vs, scale=scale).cpu().numpy()
+        ws = mode(ctr, name)
+        staged, vs = ctr.filter_projections(_t(dev, sino), g, ctr.RAM_LAK, nfft, dt, fdk=False,
+                                            ws=ws)
+        img = ctr.backproject(staged, g, dt, view_scale=vs, scale=scale, ws=ws).cpu().numpy()
         check("par2d_256", name, metrics(img, ref))
 
 
@@ -96,9 +98,9 @@ def test_accuracy_fan2d_512(ctr, orc, dev):
     ref = orc.backproject_cone(q, g.source_distance, g.detector_distance, g.detector_width,
                                g.volume_width, 1.0, 0.0, scale, (g.X, 1, g.Z))
     for name, dt in forms(ctr):
-        mode(ctr, name)
-        staged, vs = ctr.filter_projections(sino, g, ctr.COSINE, nfft, dt, fdk=True)
-        rec = ctr.backproject(staged, g, dt, view_scale=vs, scale=scale).cpu().numpy()
+        ws = mode(ctr, name)
+        staged, vs = ctr.filter_projections(sino, g, ctr.COSINE, nfft, dt, fdk=True, ws=ws)
+        rec = ctr.backproject(staged, g, dt, view_scale=vs, scale=scale, ws=ws).cpu().numpy()
         check("fan2d_512", name, metrics(rec, ref))
 
 
@@ -119,9 +121,9 @@ def test_accuracy_fdk_512(ctr, orc, dev):
              ((80, 112), (440, 472), (128, 160)))    # off axis, high cone angle
     got = {}
     for name, dt in forms(ctr):
-        mode(ctr, name)
-        staged, vs = ctr.filter_projections(proj, g, ctr.RAM_LAK, nfft, dt, fdk=True)
-        rec = ctr.backproject(staged, g, dt, view_scale=vs, scale=scale)
+        ws = mode(ctr, name)
+        staged, vs = ctr.filter_projections(proj, g, ctr.RAM_LAK, nfft, dt, fdk=True, ws=ws)
+        rec = ctr.backproject(staged, g, dt, view_scale=vs, scale=scale, ws=ws)
         got[name] = [rec[k0:k1, j0:j1, i0:i1].cpu().numpy()
                      for (i0, i1), (j0, j1), (k0, k1) in boxes]
         del staged, rec

@@ -113,21 +113,19 @@ def test_c3_storage_forms_vs_oracle(ctr, orc, dev):
     forms = (("fp32", ctr.F32), ("fp16", ctr.F16), ("bf16", ctr.BF16),
              ("fp16/fast", ctr.F16), ("bf16/fast", ctr.BF16))
     got, filtered = {}, None
-    start = ctr.get_interpolation()
-    try:
-        for name, dt in forms:
-            ctr.set_interpolation(ctr.INTERP_FAST if name.endswith("fast") else ctr.INTERP_FP32)
-            staged, vs = ctr.filter_projections(proj, g, ctr.RAM_LAK, 4096, dt, fdk=True)
-            vol = ctr.backproject(staged, g, dt, view_scale=vs, scale=scale)
-            got[name] = _crop(vol, box)
-            if dt == ctr.F32:
-                pu, pv, _ = ctr.staged_layout(g.nu, g.nv, g.n_views)
-                st = staged.view(torch.float32).view(g.n_views, pv, pu)
-                filtered = st[:, 1:g.nv + 1, 1:g.nu + 1].cpu().numpy()
-            del staged, vol
-            torch.cuda.synchronize()
-    finally:
-        ctr.set_interpolation(start)
+    ws = ctr.Workspace()
+    for name, dt in forms:
+        ws.set_interpolation(ctr.INTERP_FAST if name.endswith("fast") else ctr.INTERP_FP32)
+        staged, vs = ctr.filter_projections(proj, g, ctr.RAM_LAK, 4096, dt, fdk=True, ws=ws)
+        vol = ctr.backproject(staged, g, dt, view_scale=vs, scale=scale, ws=ws)
+        got[name] = _crop(vol, box)
+        if dt == ctr.F32:
+            pu, pv, _ = ctr.staged_layout(g.nu, g.nv, g.n_views)
+            st = staged.view(torch.float32).view(g.n_views, pv, pu)
+            filtered = st[:, 1:g.nv + 1, 1:g.nu + 1].cpu().numpy()
+        del staged, vol
+        torch.cuda.synchronize()
+    ws.close()
     del proj
     ref = orc.backproject_cone_region(filtered, g.source_distance, g.detector_distance,
                                       g.detector_width, g.volume_width, 1.0, 0.0, scale,
@@ -182,7 +180,7 @@ def test_full_size_filter_rows_sample(ctr, orc, dev):
         assert rel_max(got, ref) < 1e-5
 
 
-def test_backproject_kernel_variants_agree(ctr, dev, monkeypatch):
+def test_backproject_kernel_variants_agree(ctr, dev):
     """The LDS-staged kernel and the direct-tap kernel agree over the whole
     256^3 volume, including the corner tiles whose projections leave the
     detector sideways in some views (zero texels there).  With FP32
@@ -197,33 +195,31 @@ def test_backproject_kernel_variants_agree(ctr, dev, monkeypatch):
     import torch
     g = ctr.config_geometry("fdk_256_s")
     q = _random_views(ctr, g, dev, 5)
-    start = ctr.get_interpolation()
-    try:
-        for mode in (ctr.INTERP_FP32, ctr.INTERP_FAST):
-            ctr.set_interpolation(mode)
-            for dt in (ctr.F32, ctr.BF16, ctr.F16):
-                staged, vs = ctr.stage_projections(q, g, dt)
-                monkeypatch.setenv("CTR_BACKPROJECT_KERNEL", "lds")
-                a = ctr.backproject(staged, g, dt, view_scale=vs, scale=0.01)
-                monkeypatch.setenv("CTR_BACKPROJECT_KERNEL", "gather")
-                b = ctr.backproject(staged, g, dt, view_scale=vs, scale=0.01)
-                torch.cuda.synchronize()
-                a, b = a.cpu().numpy(), b.cpu().numpy()
-                print(f"[variants] mode {mode} dtype {dt}: max {rel_max(a, b):.3e} "
-                      f"rmse {rel_rmse(a, b):.3e}", flush=True)
-                if mode == ctr.INTERP_FP32 or dt == ctr.F32:
-                    assert rel_max(a, b) < 5e-6, (mode, dt)
-                elif dt == ctr.F16:
-                    assert rel_max(a, b) < 2e-3, dt
-                    assert rel_rmse(a, b) < 8e-4, dt
-                else:
-                    assert rel_max(a, b) < 8e-3, dt
-                    assert rel_rmse(a, b) < 3e-3, dt
-    finally:
-        ctr.set_interpolation(start)
+    lds, gather = ctr.Workspace(), ctr.Workspace()
+    lds.set_option(ctr.OPTION_BACKPROJECT_KERNEL, ctr.KERNEL_LDS)
+    gather.set_option(ctr.OPTION_BACKPROJECT_KERNEL, ctr.KERNEL_GATHER)
+    for mode in (ctr.INTERP_FP32, ctr.INTERP_FAST):
+        lds.set_interpolation(mode)
+        gather.set_interpolation(mode)
+        for dt in (ctr.F32, ctr.BF16, ctr.F16):
+            staged, vs = ctr.stage_projections(q, g, dt)
+            a = ctr.backproject(staged, g, dt, view_scale=vs, scale=0.01, ws=lds)
+            b = ctr.backproject(staged, g, dt, view_scale=vs, scale=0.01, ws=gather)
+            torch.cuda.synchronize()
+            a, b = a.cpu().numpy(), b.cpu().numpy()
+            print(f"[variants] mode {mode} dtype {dt}: max {rel_max(a, b):.3e} "
+                  f"rmse {rel_rmse(a, b):.3e}", flush=True)
+            if mode == ctr.INTERP_FP32 or dt == ctr.F32:
+                assert rel_max(a, b) < 5e-6, (mode, dt)
+            elif dt == ctr.F16:
+                assert rel_max(a, b) < 2e-3, dt
+                assert rel_rmse(a, b) < 8e-4, dt
+            else:
+                assert rel_max(a, b) < 8e-3, dt
+                assert rel_rmse(a, b) < 3e-3, dt
 
 
-def test_lds_kernel_falls_back_on_wide_footprints(ctr, orc, dev, monkeypatch):
+def test_lds_kernel_falls_back_on_wide_footprints(ctr, orc, dev):
     """High magnification: tile footprints wider than the LDS row take the
     direct-tap path inside the LDS kernel; parity is unchanged."""
     import torch
@@ -234,8 +230,9 @@ def test_lds_kernel_falls_back_on_wide_footprints(ctr, orc, dev, monkeypatch):
     qh = rng.standard_normal((g.n_views, g.nv, g.nu)).astype(np.float32)
     q = torch.from_numpy(qh).to(dev)
     staged, _ = ctr.stage_projections(q, g, ctr.F32)
-    monkeypatch.setenv("CTR_BACKPROJECT_KERNEL", "lds")
-    vol = ctr.backproject(staged, g, ctr.F32, scale=0.01).cpu().numpy()
+    ws = ctr.Workspace()
+    ws.set_option(ctr.OPTION_BACKPROJECT_KERNEL, ctr.KERNEL_LDS)
+    vol = ctr.backproject(staged, g, ctr.F32, scale=0.01, ws=ws).cpu().numpy()
     box = ((20, 36), (0, 64), (20, 36))
     ref = orc.backproject_cone_region(qh, g.source_distance, g.detector_distance,
                                       g.detector_width, g.volume_width, 1.0, 0.0, 0.01,
@@ -243,7 +240,7 @@ def test_lds_kernel_falls_back_on_wide_footprints(ctr, orc, dev, monkeypatch):
     assert rel_max(vol[20:36, :, 20:36], ref) < 1e-4
 
 
-def test_full_size_forward_kernels_agree(ctr, dev, monkeypatch):
+def test_full_size_forward_kernels_agree(ctr, dev):
     """The benchmark's projections (30 ellipsoids in 512^3, 720 views of
     768^2) from the slab-staged forward projector and from the ray kernel:
     identical bit for bit at the real shape."""
@@ -252,10 +249,11 @@ def test_full_size_forward_kernels_agree(ctr, dev, monkeypatch):
     ell = ctr.random_ellipsoids(20240602, 30, (g.X, g.Y, g.Z), g.voxel, (5, 120))
     vol = torch.zeros((g.Z, g.Y, g.X), dtype=torch.float32, device=dev)
     ctr.render_ellipsoids(ell, vol, g.volume_width)
-    monkeypatch.delenv("CTR_FORWARD_KERNEL", raising=False)
-    slab = ctr.forward_project(vol, g, step=1.0)
-    monkeypatch.setenv("CTR_FORWARD_KERNEL", "ray")
-    ray = ctr.forward_project(vol, g, step=1.0)
+    slab_ws, ray_ws = ctr.Workspace(), ctr.Workspace()
+    slab_ws.set_option(ctr.OPTION_FORWARD_KERNEL, ctr.KERNEL_AUTO)
+    ray_ws.set_option(ctr.OPTION_FORWARD_KERNEL, ctr.KERNEL_RAY)
+    slab = ctr.forward_project(vol, g, step=1.0, ws=slab_ws)
+    ray = ctr.forward_project(vol, g, step=1.0, ws=ray_ws)
     torch.cuda.synchronize()
     assert float(ray.abs().max()) > 0.0
     assert torch.equal(slab, ray)

@@ -31,10 +31,8 @@ def _cone(ctr, **kw):
 
 @pytest.fixture
 def interp(ctr):
-    """Sets the mode for one test and restores the previous one after it."""
-    start = ctr.get_interpolation()
-    yield ctr.set_interpolation
-    ctr.set_interpolation(start)
+    """A workspace whose interpolation arithmetic the test sets."""
+    return ctr.Workspace()
 
 
 def dequantize(ctr, staged, vs, g, dt):
@@ -65,10 +63,12 @@ def test_fp32_interpolation_vs_dequantized_oracle(ctr, orc, dev, interp, X, Y, Z
     for dt, tol in ((ctr.F16, 1e-3), (ctr.BF16, 4e-3)):
         staged, vs = ctr.stage_projections(p, g, dt)
         ref = _ref(orc, g, dequantize(ctr, staged, vs, g, dt), 0.01)
-        interp(ctr.INTERP_FP32)
-        strict = ctr.backproject(staged, g, dt, view_scale=vs, scale=0.01).cpu().numpy()
-        interp(ctr.INTERP_FAST)
-        fast = ctr.backproject(staged, g, dt, view_scale=vs, scale=0.01).cpu().numpy()
+        interp.set_interpolation(ctr.INTERP_FP32)
+        strict = ctr.backproject(staged, g, dt, view_scale=vs, scale=0.01,
+                                 ws=interp).cpu().numpy()
+        interp.set_interpolation(ctr.INTERP_FAST)
+        fast = ctr.backproject(staged, g, dt, view_scale=vs, scale=0.01,
+                               ws=interp).cpu().numpy()
         e_strict, e_fast = rel_max(strict, ref), rel_rmse(fast, ref)
         print(f"[interp] {X}x{Y}x{Z}x{n_views} dt={dt}: fp32 max-rel {e_strict:.2e}, "
               f"fast rel-rmse {e_fast:.2e}", flush=True)
@@ -92,10 +92,12 @@ def test_fp32_interpolation_parallel_vs_dequantized_oracle(ctr, orc, dev, interp
         q = dequantize(ctr, staged, vs, g, dt)
         ref = orc.backproject_parallel(q, g.detector_width, g.volume_width, g.rotation_count,
                                        g.rotation_offset, 0.02, (g.X, g.Y, g.Z))
-        interp(ctr.INTERP_FP32)
-        strict = ctr.backproject(staged, g, dt, view_scale=vs, scale=0.02).cpu().numpy()
-        interp(ctr.INTERP_FAST)
-        fast = ctr.backproject(staged, g, dt, view_scale=vs, scale=0.02).cpu().numpy()
+        interp.set_interpolation(ctr.INTERP_FP32)
+        strict = ctr.backproject(staged, g, dt, view_scale=vs, scale=0.02,
+                                 ws=interp).cpu().numpy()
+        interp.set_interpolation(ctr.INTERP_FAST)
+        fast = ctr.backproject(staged, g, dt, view_scale=vs, scale=0.02,
+                               ws=interp).cpu().numpy()
         e_strict, e_fast = rel_max(strict, ref), rel_rmse(fast, ref)
         print(f"[interp] parallel dt={dt}: fp32 max-rel {e_strict:.2e}, "
               f"fast rel-rmse {e_fast:.2e}", flush=True)
@@ -104,22 +106,74 @@ def test_fp32_interpolation_parallel_vs_dequantized_oracle(ctr, orc, dev, interp
 
 
 def test_packed_follows_interpolation_mode(ctr, dev, interp):
+    """Packed launches read the workspace's mode as single launches do, and
+    stay bit-identical to them under both modes."""
     import torch
     gs = [_cone(ctr), _cone(ctr, X=64, Y=64, Z=64, nu=96, nv=80, detector_width=180.0,
                             volume_width=64.0, source_distance=200.0,
                             detector_distance=150.0)]
-    interp(ctr.INTERP_FP32)
-    for dt in (ctr.F16, ctr.BF16):
-        items, singles = [], []
-        for q, g in enumerate(gs):
-            gen = torch.Generator(device=dev)
-            gen.manual_seed(40 + q)
-            p = torch.randn((g.n_views, g.nv, g.nu), generator=gen, device=dev)
-            staged, vs = ctr.stage_projections(p, g, dt)
-            vol = torch.empty((g.Z, g.Y, g.X), device=dev)
-            items.append((staged, g, vs, 0.02, vol, False))
-            singles.append(ctr.backproject(staged, g, dt, view_scale=vs, scale=0.02))
-        ctr.backproject_packed(items, dt)
-        torch.cuda.synchronize()
-        for it, ref in zip(items, singles):
-            assert torch.equal(it[4], ref), dt
+    for m in (ctr.INTERP_FP32, ctr.INTERP_FAST):
+        interp.set_interpolation(m)
+        for dt in (ctr.F16, ctr.BF16):
+            items, singles = [], []
+            for q, g in enumerate(gs):
+                gen = torch.Generator(device=dev)
+                gen.manual_seed(40 + q)
+                p = torch.randn((g.n_views, g.nv, g.nu), generator=gen, device=dev)
+                staged, vs = ctr.stage_projections(p, g, dt)
+                vol = torch.empty((g.Z, g.Y, g.X), device=dev)
+                items.append((staged, g, vs, 0.02, vol, False))
+                singles.append(ctr.backproject(staged, g, dt, view_scale=vs, scale=0.02,
+                                               ws=interp))
+            ctr.backproject_packed(items, dt, ws=interp)
+            torch.cuda.synchronize()
+            for it, ref in zip(items, singles):
+                assert torch.equal(it[4], ref), (m, dt)
+
+
+def test_two_threads_two_modes_concurrently(ctr, dev):
+    """Two threads, each with its own stream and workspace, backproject the
+    same 16-bit data at the same time, one under FP32 interpolation and one
+    under the 16-bit arithmetic: each result equals its mode's result made
+    alone, bit for bit (the mode is per workspace, not per process)."""
+    import threading
+    import torch
+    g = _cone(ctr, X=64, Y=64, Z=64, nu=96, nv=80, detector_width=180.0, volume_width=64.0,
+              source_distance=200.0, detector_distance=150.0)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(91)
+    p = torch.randn((g.n_views, g.nv, g.nu), generator=gen, device=dev)
+    staged, vs = ctr.stage_projections(p, g, ctr.F16)
+    wss = {m: ctr.Workspace() for m in (ctr.INTERP_FP32, ctr.INTERP_FAST)}
+    for m, w in wss.items():
+        w.set_interpolation(m)
+    alone = {m: ctr.backproject(staged, g, ctr.F16, view_scale=vs, scale=0.02, ws=w)
+             for m, w in wss.items()}
+    torch.cuda.synchronize()
+    assert not torch.equal(alone[ctr.INTERP_FP32], alone[ctr.INTERP_FAST])
+    results = {m: [] for m in wss}
+    errors = []
+    barrier = threading.Barrier(len(wss))
+
+    def worker(m):
+        try:
+            s = torch.cuda.Stream()
+            barrier.wait()
+            with torch.cuda.stream(s):
+                for _ in range(6):
+                    results[m].append(ctr.backproject(staged, g, ctr.F16, view_scale=vs,
+                                                      scale=0.02, ws=wss[m], stream=s))
+            s.synchronize()
+        except Exception as e:  # reported by the main thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(m,)) for m in wss]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=60)
+    assert not errors, errors
+    for m in wss:
+        assert len(results[m]) == 6
+        for r in results[m]:
+            assert torch.equal(r, alone[m]), m

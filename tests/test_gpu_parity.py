@@ -183,7 +183,7 @@ def test_forward_project_vs_oracle(ctr, orc, dev, kind):
 
 
 @pytest.mark.parametrize("kind", ["cone", "parallel"])
-def test_forward_project_interior_vs_oracle(ctr, orc, dev, monkeypatch, kind):
+def test_forward_project_interior_vs_oracle(ctr, orc, dev, kind):
     """A volume large enough that most samples of a ray lie inside it and
     take the unchecked interior loop (the samples near the faces still take
     the checked one): against the oracle's ray-marcher, and the slab-staged
@@ -204,21 +204,29 @@ def test_forward_project_interior_vs_oracle(ctr, orc, dev, monkeypatch, kind):
                          detector_width=96.0, volume_width=64.0, rotation_count=0.5)
         ref = orc.forward_parallel(vol, g.volume_width, g.detector_width, g.nu, g.nv,
                                    g.n_views, 0.5, 0.0, 0.5)
-    got = ctr.forward_project(_t(dev, vol), g, step=0.5)
+    got = ctr.forward_project(_t(dev, vol), g, step=0.5, ws=_forward_ws(ctr, False, False))
     assert rel_max(got.cpu().numpy(), ref) < 1e-4
-    monkeypatch.setenv("CTR_FORWARD_KERNEL", "ray")
-    ray = ctr.forward_project(_t(dev, vol), g, step=0.5)
+    ray = ctr.forward_project(_t(dev, vol), g, step=0.5, ws=_forward_ws(ctr, True, False))
     assert torch.equal(ray, got)
-    monkeypatch.setenv("CTR_FORWARD_ADDRESSING", "pointer")
-    wide = ctr.forward_project(_t(dev, vol), g, step=0.5)
+    wide = ctr.forward_project(_t(dev, vol), g, step=0.5, ws=_forward_ws(ctr, True, True))
     assert torch.equal(wide, got)
-    monkeypatch.delenv("CTR_FORWARD_KERNEL")
-    wide_slab = ctr.forward_project(_t(dev, vol), g, step=0.5)
+    wide_slab = ctr.forward_project(_t(dev, vol), g, step=0.5,
+                                    ws=_forward_ws(ctr, False, True))
     assert torch.equal(wide_slab, got)
 
 
+def _forward_ws(ctr, ray, pointer):
+    """Workspace choosing the forward kernel (ray or slab) and addressing
+    (64-bit pointers or the buffer resource)."""
+    ws = ctr.Workspace()
+    ws.set_option(ctr.OPTION_FORWARD_KERNEL, ctr.KERNEL_RAY if ray else ctr.KERNEL_AUTO)
+    ws.set_option(ctr.OPTION_FORWARD_ADDRESSING,
+                  ctr.KERNEL_POINTER if pointer else ctr.KERNEL_AUTO)
+    return ws
+
+
 @pytest.mark.parametrize("step", [0.5, 1.0, 2.0])
-def test_forward_slab_kernel_matches_ray_kernel(ctr, dev, monkeypatch, step):
+def test_forward_slab_kernel_matches_ray_kernel(ctr, dev, step):
     """The slab-staged forward projector against the ray kernel, bit for bit:
     a full turn of views (every main axis and direction), a fan wide enough
     that some rays run against the view's central direction (those take the
@@ -237,14 +245,9 @@ def test_forward_slab_kernel_matches_ray_kernel(ctr, dev, monkeypatch, step):
                           detector_width=64.0, volume_width=45.0, rotation_count=0.5,
                           rotation_offset=0.2)]
     for g in geoms:
-        monkeypatch.delenv("CTR_FORWARD_KERNEL", raising=False)
-        monkeypatch.delenv("CTR_FORWARD_ADDRESSING", raising=False)
-        slab = ctr.forward_project(vol, g, step=step)
-        monkeypatch.setenv("CTR_FORWARD_ADDRESSING", "pointer")
-        slab_wide = ctr.forward_project(vol, g, step=step)
-        monkeypatch.setenv("CTR_FORWARD_KERNEL", "ray")
-        monkeypatch.delenv("CTR_FORWARD_ADDRESSING")
-        ray = ctr.forward_project(vol, g, step=step)
+        slab = ctr.forward_project(vol, g, step=step, ws=_forward_ws(ctr, False, False))
+        slab_wide = ctr.forward_project(vol, g, step=step, ws=_forward_ws(ctr, False, True))
+        ray = ctr.forward_project(vol, g, step=step, ws=_forward_ws(ctr, True, False))
         assert float(ray.abs().max()) > 0.0
         assert torch.equal(slab, ray), (g.kind, g.nu, step)
         assert torch.equal(slab_wide, ray), (g.kind, g.nu, step)
@@ -279,7 +282,7 @@ def test_error_paths(ctr, dev):
 
 
 @pytest.mark.parametrize("X,Y,Z,n_views", [(37, 45, 29, 130), (33, 64, 18, 1)])
-def test_lds_kernel_edge_shapes(ctr, orc, dev, monkeypatch, X, Y, Z, n_views):
+def test_lds_kernel_edge_shapes(ctr, orc, dev, X, Y, Z, n_views):
     """The LDS-staged kernel on ragged tiles (X, Z not multiples of 16), a
     partial last y-run (Y not a multiple of 32), a view count that crosses
     the 128-view refill of the rectangle table, and a single view; every
@@ -288,10 +291,11 @@ def test_lds_kernel_edge_shapes(ctr, orc, dev, monkeypatch, X, Y, Z, n_views):
     rng = np.random.default_rng(X * Y + n_views)
     q = rng.standard_normal((g.n_views, g.nv, g.nu)).astype(np.float32)
     ref = _fdk_ref(orc, g, q, 0.01)
-    monkeypatch.setenv("CTR_BACKPROJECT_KERNEL", "lds")
+    ws = ctr.Workspace()
+    ws.set_option(ctr.OPTION_BACKPROJECT_KERNEL, ctr.KERNEL_LDS)
     for dt, tol in ((ctr.F32, 1e-4), (ctr.F16, 1e-3), (ctr.BF16, 4e-3)):
         staged, vs = ctr.stage_projections(_t(dev, q), g, dt)
-        got = ctr.backproject(staged, g, dt, view_scale=vs, scale=0.01).cpu().numpy()
+        got = ctr.backproject(staged, g, dt, view_scale=vs, scale=0.01, ws=ws).cpu().numpy()
         if dt == ctr.F32:
             assert rel_max(got, ref) < tol, dt
         else:
