@@ -78,3 +78,60 @@ def test_cpp_fbp3d_reports_reference_errors(ctr, tmp_path):
                        text=True, timeout=60)
     assert r.returncode == 1
     assert "[CTAnalyzer Error]" in (r.stdout + r.stderr)
+
+
+def test_cpp_fbp3d_slice_and_view_copies(ctr, dev, tmp_path):
+    """FBP3D's slice and view copies (FBP3D.cpp:639-778) on the drop-in
+    class: every load into an exact-size and an odd-size texture, every store
+    from both, against numpy index identities.  The copied extent is the min
+    of both sizes; the rest of a target keeps its value (-1 for loads)."""
+    exe = os.path.join(os.path.dirname(ctr.LIB_PATH), "examples", "fbp3d_copies_main")
+    X, Y, Z, nu, nv, N = 20, 12, 16, 24, 10, 9
+    rng = np.random.default_rng(4)
+    vol = rng.standard_normal((Z, Y, X)).astype(np.float32)
+    proj = rng.standard_normal((N, nv, nu)).astype(np.float32)
+    sino = rng.standard_normal((nv, N, nu)).astype(np.float32)
+    patch = rng.standard_normal(4096).astype(np.float32)
+    for name, a in (("volume", vol), ("projections", proj), ("sinograms", sino),
+                    ("patch", patch)):
+        a.tofile(tmp_path / f"{name}.raw")
+    r = subprocess.run([exe, str(tmp_path), *map(str, (X, Y, Z, nu, nv, N))],
+                       capture_output=True, text=True, timeout=120)
+    # the program ends on the reference's out-of-range error, after its dumps
+    assert r.returncode == 1 and "load_projection: index out of range" in r.stdout + r.stderr
+
+    def got(name, h, w):
+        return np.fromfile(tmp_path / f"{name}.raw", dtype=np.float32).reshape(h, w)
+
+    sources = {"slice_x": (vol[:, :, X // 3], Y, Z), "slice_y": (vol[:, Y // 2, :], X, Z),
+               "slice_z": (vol[Z - 1], X, Y), "projection": (proj[N // 2], nu, nv),
+               "sinogram": (sino[nv // 3], nu, N),
+               "projection_from_sinograms": (sino[:, 1, :], nu, nv)}
+    for name, (src, w, h) in sources.items():
+        assert np.array_equal(got(name + "_exact", h, w), src), name
+        odd = got(name + "_odd", h - 2, w + 3)
+        ref = np.full((h - 2, w + 3), -1.0, dtype=np.float32)
+        ref[:, :w] = src[:h - 2, :w]
+        assert np.array_equal(odd, ref), name
+
+    def P(h, w):
+        return patch[:h * w].reshape(h, w)
+
+    v = vol.copy()
+    v[:, :, 1] = P(Z, Y)
+    v[:Z, :Y - 2, X - 1] = P(Z + 1, Y - 2)[:Z]
+    v[:, 0, :] = P(Z, X)
+    v[:Z - 3, Y - 2, :] = P(Z - 3, X + 2)[:, :X]
+    v[2] = P(Y, X)
+    v[Z // 2, :, :X - 1] = P(Y + 2, X - 1)[:Y]
+    assert np.array_equal(got("volume_out", Z * Y, X).reshape(Z, Y, X), v)
+    p = proj.copy()
+    p[0] = P(nv, nu)
+    p[N - 1, :nv - 1] = P(nv - 1, nu + 1)[:, :nu]
+    assert np.array_equal(got("projections_out", N * nv, nu).reshape(N, nv, nu), p)
+    s = sino.copy()
+    s[0] = P(N, nu)
+    s[nv - 1, :, :nu - 2] = P(N + 1, nu - 2)[:N]
+    s[:, 2, :] = P(nv, nu)
+    s[:nv - 2, N - 2, :] = P(nv - 2, nu + 2)[:, :nu]
+    assert np.array_equal(got("sinograms_out", nv * N, nu).reshape(nv, N, nu), s)

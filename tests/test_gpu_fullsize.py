@@ -257,3 +257,40 @@ def test_full_size_forward_kernels_agree(ctr, dev):
     torch.cuda.synchronize()
     assert float(ray.abs().max()) > 0.0
     assert torch.equal(slab, ray)
+
+
+@pytest.mark.parametrize("Y", [64, 8])
+def test_2048_panels_vs_oracle(ctr, orc, dev, Y):
+    """2048 x 2048 panels, the reference's native scan frames
+    (Mains/fbp_cone_fdk_main.cpp:23): a staged view is 2080 x 2051 texels
+    (17 MB), past 2^22 texels.  A 2048 x Y x 2048 slab at 0.1 mm (1 detector
+    pixel per voxel at magnification 2) from 16 views, Y = 64 through the
+    LDS kernel and Y = 8 through the direct-tap kernel, FP32 and FP16
+    storage, against the FP64 oracle on crops at the centre and near a
+    corner of the field."""
+    import torch
+    g = ctr.Geometry(ctr.CONE, nu=2048, nv=2048, n_views=16, X=2048, Y=Y, Z=2048,
+                     detector_width=409.6, volume_width=204.8, source_distance=500.0,
+                     detector_distance=500.0, rotation_offset=0.1)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(2048 + Y)
+    q = torch.randn((g.n_views, g.nv, g.nu), generator=gen, device=dev)
+    qh = q.cpu().numpy()
+    c = min(16, Y)
+    boxes = (((1016, 1032), (Y // 2 - c // 2, Y // 2 + c // 2), (1016, 1032)),
+             ((120, 136), (0, c), (1890, 1906)))
+    refs = [orc.backproject_cone_region(qh, g.source_distance, g.detector_distance,
+                                        g.detector_width, g.volume_width, 1.0, 0.1, 0.01,
+                                        (g.X, g.Y, g.Z), box) for box in boxes]
+    for dt in (ctr.F32, ctr.F16):
+        staged, vs = ctr.stage_projections(q, g, dt)
+        vol = ctr.backproject(staged, g, dt, view_scale=vs, scale=0.01)
+        for box, ref in zip(boxes, refs):
+            got = _crop(vol, box)
+            assert np.abs(ref).max() > 0
+            if dt == ctr.F32:
+                assert rel_max(got, ref) < 1e-4, (Y, box)
+            else:
+                assert rel_rmse(got, ref) < 1e-3, (Y, box)
+        del staged, vol
+        torch.cuda.synchronize()
