@@ -88,6 +88,7 @@ SIGNATURES = {
     "ctr_rebin_fan_to_parallel": [_v, _v, _i, _i, _d, _d, _d, _d, _v, _i, _d],
     "ctr_render_ellipsoids": [_v, _v, _v, _i, _v, _i, _i, _i, _d],
     "ctr_poisson_log": [_v, _v, _sz, _d, _u],
+    "ctr_project_ellipsoids": [_v, _i, _v, _i, _d, _d, _d, _d, _d, _v, _i, _i, _i, _i, _i],
     "ctr_decode_frames": [_v, _v, _i, _i, _i, _i, _i, _v, _i, _i, _i, _i, _i],
     "ctr_min_max": [_v, _v, _v, _sz, _v],
     "ctr_normalize_range": [_v, _v, _sz, _v],
@@ -409,6 +410,26 @@ def render_ellipsoids(ellipsoids, volume, volume_width, ws=None, stream=None):
         e.ctypes.data_as(ctypes.c_void_p), e.shape[0], ptr(volume), X, Y, Z,
         float(volume_width)))
     return volume
+
+
+def project_ellipsoids(ellipsoids, geom, layout=PROJECTIONS, out=None, accumulate=False,
+                       device=None, stream=None):
+    """Exact line integrals of ellipsoid records (ctr_project_ellipsoids)
+    over the detector rays of `geom`; returns `out`, a float32 (N, nv, nu)
+    or (nv, N, nu) device tensor."""
+    torch = _torch()
+    e = np.ascontiguousarray(ellipsoids, dtype=np.float64).reshape(-1, 10)
+    if out is None:
+        shape = ((geom.n_views, geom.nv, geom.nu) if layout == PROJECTIONS
+                 else (geom.nv, geom.n_views, geom.nu))
+        out = torch.empty(shape, dtype=torch.float32, device=device or "cuda")
+    _f32c(out)
+    check(lib().ctr_project_ellipsoids(
+        stream_handle(stream), geom.kind, e.ctypes.data_as(ctypes.c_void_p), e.shape[0],
+        geom.source_distance, geom.detector_distance, geom.detector_width,
+        geom.rotation_count, geom.rotation_offset, ptr(out), layout, geom.nu, geom.nv,
+        geom.n_views, 1 if accumulate else 0))
+    return out
 
 
 def poisson_log(proj, i0, seed, stream=None):

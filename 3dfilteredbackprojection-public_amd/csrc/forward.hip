@@ -713,6 +713,136 @@ __global__ void poisson_log_kernel(float *p, long long count, double i0, unsigne
     p[idx] = (float)(-log(fmax(k, 1.0) / i0));
 }
 
+// Host side of both ellipsoid generators: a record of 10 doubles (centre,
+// semi-axes, angles alpha beta gamma, density) -> centre, the rows of R^T
+// divided by the semi-axes (world -> unit-sphere body frame), with
+// R = Rz(gamma) Ry(beta) Rx(alpha), and the density.
+static int prepare_ellipsoid(const double *P, int e, EllipsoidRec *Q)
+{
+    CTR_REQUIRE(P[3] > 0 && P[4] > 0 && P[5] > 0, CTR_ERR_INVALID_ARGUMENT,
+                "ellipsoid %d has a non-positive semi-axis", e);
+    double ca = cos(P[6]), sa = sin(P[6]), cb = cos(P[7]), sb = sin(P[7]);
+    double cg = cos(P[8]), sg = sin(P[8]);
+    double Rx[3][3] = {{1, 0, 0}, {0, ca, -sa}, {0, sa, ca}};
+    double Ry[3][3] = {{cb, 0, sb}, {0, 1, 0}, {-sb, 0, cb}};
+    double Rz[3][3] = {{cg, -sg, 0}, {sg, cg, 0}, {0, 0, 1}};
+    double T[3][3], R[3][3];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            T[r][c] = 0;
+            for (int m = 0; m < 3; m++) T[r][c] += Ry[r][m] * Rx[m][c];
+        }
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            R[r][c] = 0;
+            for (int m = 0; m < 3; m++) R[r][c] += Rz[r][m] * T[m][c];
+        }
+    for (int a = 0; a < 3; a++) {
+        Q->c[a] = P[a];
+        for (int b = 0; b < 3; b++) Q->m[3 * a + b] = R[b][a] / P[3 + a];
+    }
+    Q->density = P[9];
+    return CTR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Analytic projections of ellipsoids: the exact line integral
+// sum_e density_e * chord_e along each detector ray, the rays of
+// ctr_forward_project (source SOD (cos t, 0, sin t), flat detector at the
+// detector distance beyond the axis; parallel rays along -(cos t, 0, sin t)
+// through u (-sin t, 0, cos t) + v y).  One thread per ray.  Each ellipsoid
+// is first culled in FP32 against its bounding sphere (distance of the
+// centre from the ray, with a margin far above FP32 rounding); the chord of
+// a candidate is FP64: the ray is moved to its point nearest the centre,
+// taken to the unit-sphere body frame, and the chord is
+// 2 sqrt(B^2 - A C) / A of |r + s q|^2 = 1.  The records travel in the kernel
+// argument (no device table, no host wait): at most kEllGroup per launch.
+// ---------------------------------------------------------------------------
+constexpr int kEllGroup = 28;
+struct EllipsoidGroup {
+    double c[kEllGroup][3];
+    double m[kEllGroup][9];
+    double density[kEllGroup];
+    float cf[kEllGroup][3];
+    float reach2[kEllGroup];  // (bounding radius + margin)^2, FP32 cull
+    int count;
+};
+struct RayGeometry {
+    int geometry, layout, nu, nv, n_views, accumulate;
+    double sod, odd, delta, rot, offset;
+    float *out;
+};
+static_assert(sizeof(EllipsoidGroup) + sizeof(RayGeometry) < 4000, "kernel argument size");
+
+__global__ __launch_bounds__(256) void project_ellipsoids_kernel(EllipsoidGroup E, RayGeometry G)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)G.nu * G.nv * G.n_views;
+    if (idx >= total) return;
+    const int iu = (int)(idx % G.nu);
+    const long long r = idx / G.nu;
+    int iv, n;
+    if (G.layout == CTR_LAYOUT_PROJECTIONS) {
+        iv = (int)(r % G.nv);
+        n = (int)(r / G.nv);
+    } else {
+        n = (int)(r % G.n_views);
+        iv = (int)(r / G.n_views);
+    }
+    // FBP3D.cpp:223-227,283-284: theta_n = 2 pi rot n / N + offset
+    const double th = 2.0 * M_PI * G.rot * (double)n / (double)G.n_views + G.offset;
+    double st, ct;
+    sincos(th, &st, &ct);
+    const double u = (iu + 0.5 - 0.5 * G.nu) * G.delta, v = (iv + 0.5 - 0.5 * G.nv) * G.delta;
+    double o[3], d[3];
+    if (G.geometry == CTR_GEOMETRY_CONE) {
+        o[0] = G.sod * ct;
+        o[1] = 0.0;
+        o[2] = G.sod * st;
+        d[0] = -G.odd * ct - u * st - o[0];
+        d[1] = v;
+        d[2] = -G.odd * st + u * ct - o[2];
+    } else {
+        o[0] = -u * st;
+        o[1] = v;
+        o[2] = u * ct;
+        d[0] = -ct;
+        d[1] = 0.0;
+        d[2] = -st;
+    }
+    const double inv = 1.0 / sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    d[0] *= inv;
+    d[1] *= inv;
+    d[2] *= inv;
+    const float of[3] = {(float)o[0], (float)o[1], (float)o[2]};
+    const float df[3] = {(float)d[0], (float)d[1], (float)d[2]};
+    double acc = 0.0;
+    for (int e = 0; e < E.count; e++) {
+        // cull: squared distance of the centre from the ray, FP32
+        const float wx = E.cf[e][0] - of[0], wy = E.cf[e][1] - of[1], wz = E.cf[e][2] - of[2];
+        const float t = wx * df[0] + wy * df[1] + wz * df[2];
+        const float h2 = wx * wx + wy * wy + wz * wz - t * t;
+        if (h2 > E.reach2[e]) continue;
+        // FP64 chord: p = point of the ray nearest the centre, minus the centre
+        const double w0 = E.c[e][0] - o[0], w1 = E.c[e][1] - o[1], w2 = E.c[e][2] - o[2];
+        const double t0 = w0 * d[0] + w1 * d[1] + w2 * d[2];
+        const double p0 = t0 * d[0] - w0, p1 = t0 * d[1] - w1, p2 = t0 * d[2] - w2;
+        const double *M = E.m[e];
+        double A = 0.0, B = 0.0, C = -1.0;
+        for (int a = 0; a < 3; a++) {
+            const double ra = M[3 * a] * p0 + M[3 * a + 1] * p1 + M[3 * a + 2] * p2;
+            const double qa = M[3 * a] * d[0] + M[3 * a + 1] * d[1] + M[3 * a + 2] * d[2];
+            A += qa * qa;
+            B += ra * qa;
+            C += ra * ra;
+        }
+        const double disc = B * B - A * C;
+        if (disc > 0.0) acc += E.density[e] * 2.0 * sqrt(disc) / A;
+    }
+    float *dst = G.out + idx;
+    *dst = G.accumulate ? *dst + (float)acc : (float)acc;
+}
+
 }  // namespace ctr
 
 using namespace ctr;
@@ -876,32 +1006,8 @@ int ctr_render_ellipsoids(void *stream, ctr_workspace *ws, const double *ellipso
     if (count == 0) return CTR_OK;
     std::vector<EllipsoidRec> recs(count);
     for (int e = 0; e < count; e++) {
-        const double *P = ellipsoids + 10 * (size_t)e;
-        CTR_REQUIRE(P[3] > 0 && P[4] > 0 && P[5] > 0, CTR_ERR_INVALID_ARGUMENT,
-                    "ellipsoid %d has a non-positive semi-axis", e);
-        double ca = cos(P[6]), sa = sin(P[6]), cb = cos(P[7]), sb = sin(P[7]);
-        double cg = cos(P[8]), sg = sin(P[8]);
-        // R = Rz(g) Ry(b) Rx(a)
-        double Rx[3][3] = {{1, 0, 0}, {0, ca, -sa}, {0, sa, ca}};
-        double Ry[3][3] = {{cb, 0, sb}, {0, 1, 0}, {-sb, 0, cb}};
-        double Rz[3][3] = {{cg, -sg, 0}, {sg, cg, 0}, {0, 0, 1}};
-        double T[3][3], R[3][3];
-        for (int r = 0; r < 3; r++)
-            for (int c = 0; c < 3; c++) {
-                T[r][c] = 0;
-                for (int m = 0; m < 3; m++) T[r][c] += Ry[r][m] * Rx[m][c];
-            }
-        for (int r = 0; r < 3; r++)
-            for (int c = 0; c < 3; c++) {
-                R[r][c] = 0;
-                for (int m = 0; m < 3; m++) R[r][c] += Rz[r][m] * T[m][c];
-            }
-        EllipsoidRec &Q = recs[e];
-        for (int a = 0; a < 3; a++) {
-            Q.c[a] = P[a];
-            for (int b = 0; b < 3; b++) Q.m[3 * a + b] = R[b][a] / P[3 + a];
-        }
-        Q.density = P[9];
+        int rc = prepare_ellipsoid(ellipsoids + 10 * (size_t)e, e, &recs[e]);
+        if (rc) return rc;
     }
     ctr_workspace *w = nullptr;
     int rc = workspace_resolve(ws, &w);
@@ -917,6 +1023,69 @@ int ctr_render_ellipsoids(void *stream, ctr_workspace *ws, const double *ellipso
     CTR_CHECK_LAUNCH();
     // the host records must outlive the copy
     CTR_TRY_HIP(hipStreamSynchronize(s));
+    return CTR_OK;
+}
+
+int ctr_project_ellipsoids(void *stream, int geometry, const double *ellipsoids, int count,
+                           double source_distance, double detector_distance,
+                           double detector_width, double rotation_count, double rotation_offset,
+                           float *out, int layout, int nu, int nv, int n_views, int accumulate)
+{
+    clear_error();
+    CTR_REQUIRE(out && (ellipsoids || count == 0), CTR_ERR_NULL_POINTER,
+                "ctr_project_ellipsoids: NULL buffer");
+    CTR_REQUIRE(count >= 0 && count <= 4096, CTR_ERR_INVALID_ARGUMENT,
+                "ctr_project_ellipsoids: count %d outside [0, 4096]", count);
+    CTR_REQUIRE(geometry == CTR_GEOMETRY_CONE || geometry == CTR_GEOMETRY_PARALLEL,
+                CTR_ERR_INVALID_ARGUMENT, "unknown geometry %d", geometry);
+    CTR_REQUIRE(layout == CTR_LAYOUT_PROJECTIONS || layout == CTR_LAYOUT_SINOGRAMS,
+                CTR_ERR_INVALID_ARGUMENT, "unknown layout %d", layout);
+    CTR_REQUIRE(nu > 0 && nv > 0 && n_views > 0 && detector_width > 0,
+                CTR_ERR_INVALID_ARGUMENT, "ctr_project_ellipsoids: bad detector");
+    CTR_REQUIRE(geometry == CTR_GEOMETRY_PARALLEL || (source_distance > 0 && detector_distance >= 0),
+                CTR_ERR_INVALID_ARGUMENT, "ctr_project_ellipsoids: bad cone geometry");
+    const long long total = (long long)nu * nv * n_views;
+    CTR_REQUIRE(total < (1LL << 40), CTR_ERR_INVALID_ARGUMENT, "too many rays");
+    RayGeometry G;
+    G.geometry = geometry;
+    G.layout = layout;
+    G.nu = nu;
+    G.nv = nv;
+    G.n_views = n_views;
+    G.sod = source_distance;
+    G.odd = detector_distance;
+    G.delta = detector_width / nu;
+    G.rot = rotation_count;
+    G.offset = rotation_offset;
+    G.out = out;
+    hipStream_t s = as_stream(stream);
+    for (int e0 = 0; e0 < count || (e0 == 0 && !accumulate); e0 += kEllGroup) {
+        EllipsoidGroup E;
+        E.count = count - e0 < kEllGroup ? count - e0 : kEllGroup;
+        for (int q = 0; q < E.count; q++) {
+            const double *P = ellipsoids + 10 * (size_t)(e0 + q);
+            EllipsoidRec R;
+            int rc = prepare_ellipsoid(P, e0 + q, &R);
+            if (rc) return rc;
+            for (int a = 0; a < 3; a++) {
+                E.c[q][a] = R.c[a];
+                E.cf[q][a] = (float)R.c[a];
+            }
+            for (int a = 0; a < 9; a++) E.m[q][a] = R.m[a];
+            E.density[q] = R.density;
+            // squared bounding radius plus a margin: the FP32 cull's h2 =
+            // |w|^2 - t^2 carries an absolute rounding of a few 2^-24 |w|^2,
+            // |w| <= `far`; 1e-6 far^2 is over 16 times that
+            const double far = fabs(R.c[0]) + fabs(R.c[1]) + fabs(R.c[2]) + source_distance +
+                               detector_distance + detector_width;
+            const double rad = fmax(P[3], fmax(P[4], P[5]));
+            E.reach2[q] = (float)(rad * rad * (1.0 + 1e-4) + 1e-6 * far * far);
+        }
+        G.accumulate = accumulate || e0 > 0;
+        project_ellipsoids_kernel<<<grid1d(total, 256), 256, 0, s>>>(E, G);
+        CTR_CHECK_LAUNCH();
+        if (count == 0) break;
+    }
     return CTR_OK;
 }
 

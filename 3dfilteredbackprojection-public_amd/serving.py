@@ -7,8 +7,12 @@ small   256^3  from  360 views of  384^2
 medium  512^3  from  720 views of  768^2
 large   1024^3 from 1200 views of 1024^2
 
-Each request carries its own projections; filtered projections are stored
-in FP16 with a per-view scale, and the volume is FP32.
+Each request carries its own seeded ellipsoid phantom: its projections are
+the phantom's exact line integrals, generated on the device when the request
+is admitted (ctr_project_ellipsoids, in the slot's stream ahead of the
+pipeline; this stands in for the upload of a scan and counts toward the
+request's latency).  Filtered projections are stored in FP16 with a per-view
+scale, and the volume is FP32.
 
 Design:
 * A *slot* owns the device buffers of one reconstruction of a class: the
@@ -39,9 +43,9 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import (BF16, CONE, F16, F32, MAX_PACKED, RAM_LAK, Workspace, backproject,
-               backproject_packed, config_geometry, filter_projections, forward_project,
-               poisson_log, random_ellipsoids, render_ellipsoids, staged_layout)
+from . import (F16, F32, MAX_PACKED, RAM_LAK, Workspace, backproject, backproject_packed,
+               config_geometry, filter_projections, project_ellipsoids, random_ellipsoids,
+               staged_layout)
 
 CLASSES = {"small": "fdk_256_s", "medium": "fdk_512_s", "large": "fdk_1024_s"}
 
@@ -67,18 +71,21 @@ class Request:
         return self.done - self.arrival
 
 
-def make_projections(geom, seed, device, count=None):
-    """Seeded ellipsoid phantom -> ray-driven projections -> Poisson(1e5) + log."""
-    import torch
-    n = count or max(4, int(30 * geom.X / 512))
-    ell = random_ellipsoids(seed, n, (geom.X, geom.Y, geom.Z), geom.voxel, (5, geom.X * 120 / 512))
+def request_phantom(geom, seed):
+    """The seeded ellipsoid phantom of one request: 30 ellipsoids per 512
+    voxels of width (15 / 30 / 60 for the three classes), semi-axes 5 to
+    120 voxels per 512, densities 0.1 .. 1.0 in units of 0.02 / mm."""
+    n = max(4, int(30 * geom.X / 512))
+    ell = random_ellipsoids(seed, n, (geom.X, geom.Y, geom.Z), geom.voxel,
+                            (5, geom.X * 120 / 512))
     ell[:, 9] *= 0.02
-    vol = torch.zeros((geom.Z, geom.Y, geom.X), dtype=torch.float32, device=device)
-    render_ellipsoids(ell, vol, geom.volume_width)
-    proj = forward_project(vol, geom, step=1.0)
-    poisson_log(proj, 1e5, seed)
-    del vol
-    return proj
+    return ell
+
+
+def make_projections(geom, seed, out=None, device=None, stream=None):
+    """Exact projections of the request phantom of `seed`."""
+    return project_ellipsoids(request_phantom(geom, seed), geom, out=out, device=device,
+                              stream=stream)
 
 
 class Slot:
@@ -96,7 +103,7 @@ class Slot:
         g = geom
         _, _, sbytes = staged_layout(g.nu, g.nv, g.n_views, dtype)
         _, _, s32 = staged_layout(g.nu, g.nv, g.n_views, F32)
-        self.proj = [torch.empty((g.n_views, g.nv, g.nu), dtype=torch.float32, device=dev)
+        self.proj = [torch.zeros((g.n_views, g.nv, g.nu), dtype=torch.float32, device=dev)
                      for _ in range(count)]
         self.staged = [torch.empty(sbytes, dtype=torch.uint8, device=dev) for _ in range(count)]
         self.vs = [torch.empty(g.n_views, dtype=torch.float32, device=dev) for _ in range(count)]
@@ -135,12 +142,12 @@ class Slot:
             backproject_packed([(self.staged[q], g, self.vs[q], self.scale, vols[q], True)
                                 for q in range(fill)], self.dtype, ws=self.ws)
 
-    def launch(self, reqs, sources, keep_results=False):
+    def launch(self, reqs, keep_results=False):
         import torch
         fill = len(reqs)
         with torch.cuda.stream(self.stream):
-            for q, src in enumerate(sources):
-                self.proj[q].copy_(src, non_blocking=True)
+            for q, r in enumerate(reqs):
+                make_projections(self.geom, r.seed, out=self.proj[q], stream=self.stream)
             if self.graphs:
                 self.graphs[fill].replay()
             else:
@@ -160,7 +167,7 @@ class Server:
     """Admission, slot management and the serving loop."""
 
     def __init__(self, budget_bytes=200e9, dtype=F16, pack=4, max_slots=None,
-                 use_graphs=True, keep_results=False, pool=2, log=None, geometries=None):
+                 use_graphs=True, keep_results=False, log=None, geometries=None):
         import torch
         self.device = torch.device("cuda")
         self.budget = float(budget_bytes)
@@ -172,25 +179,7 @@ class Server:
         self.geoms = geometries or {c: config_geometry(w) for c, w in CLASSES.items()}
         self.slots = []
         self.used = 0.0
-        self.pool_size = pool
-        self.sources = {}
         self.log = log or (lambda m: None)
-
-    # --- inputs ------------------------------------------------------------
-    def source(self, cls, seed):
-        key = (cls, seed % self.pool_size)
-        if key not in self.sources:
-            self.sources[key] = make_projections(self.geoms[cls], 1000 + 7919 * key[1] +
-                                                 {"small": 0, "medium": 1, "large": 2}[cls],
-                                                 self.device)
-        return self.sources[key]
-
-    def prepare_sources(self, classes):
-        for c in classes:
-            for s in range(self.pool_size):
-                self.source(c, s)
-        import torch
-        torch.cuda.synchronize()
 
     # --- slots -------------------------------------------------------------
     def _slot_count(self, cls):
@@ -302,8 +291,7 @@ class Server:
                 for r2 in group:
                     queue.remove(r2)
                     r2.admitted = time.perf_counter() - t0
-                slot.launch(group, [self.source(x.cls, x.seed) for x in group],
-                            self.keep_results)
+                slot.launch(group, self.keep_results)
                 progressed = True
             if not progressed:
                 nxt = pending[0].arrival - (time.perf_counter() - t0) if pending else 0.0

@@ -1,4 +1,5 @@
-"""FDK benchmark on one MI355X: 512^3 volume from 720 views of 768^2.
+"""FDK benchmark on one MI355X: 512^3 volume from 720 views of 768^2, plus the
+serving stream.
 
 A step is one full FDK pass over resident projections: FDK cosine weight +
 ramp filter (FFT length 2048) into the staged layout, then the voxel-driven
@@ -8,19 +9,32 @@ Poisson noise at I0 = 1e5 and log.  The headline value is the FP16-stored
 form (filtered projections in FP16 with a per-view scale, FP32 interpolation
 and accumulation); the FP32-stored and BF16-stored forms are timed beside it,
 and the 16-bit forms once more with the 16-bit interpolation arithmetic
-(workspace option CTR_OPTION_INTERPOLATION = CTR_INTERP_FAST) as a secondary
-figure.
+(workspace option CTR_OPTION_INTERPOLATION = CTR_INTERP_FAST).
 
-Prints ONE JSON line.  Roofline for the dominant kernel (backprojection):
-algorithmic bytes per voxel-view update = 4 taps x element size + 8/N bytes
-of volume traffic, divided by the kernel's average duration (HIP events on
-the launch stream), against 8 TB/s.  The taps are served from LDS, so that
-fraction exceeds 1; `traffic_frac` is the HBM bytes the counters measured per
-launch over the same duration, against the same peak.
+Roofline of the dominant kernel (the backprojection).  It is bound by VALU
+issue, not HBM: its four bilinear taps per update come from LDS.  So
+`roofline` is VALU issue: wave64 VALU instructions per launch (SQ_INSTS_VALU
+per voxel-view update, measured by rocprofv3 and recorded in
+profiles/backproject_fdk512_sq_counters.md, times the updates of one launch)
+over the launch's duration (HIP events on its stream), against the issue
+rate of MI355X's published 157.3 TFLOP/s FP32 vector peak: 157.3e12 / 2
+FLOP per FMA / 64 lanes = 1229 G wave-instructions/s.  That published rate
+counts packed FP32 (two FMAs per lane per instruction); without it, which
+this build does not use, the ceiling is half that.  The HBM side is reported
+too: bytes measured per launch (FETCH_SIZE + WRITE_SIZE, scaled by a copy of
+known size, profiles/backproject_fdk512_traffic.md) over the same duration,
+against 8 TB/s, and the tap bytes the LDS serves.
+
+Also in the line: the forward projector on the same scan, the CPU baseline
+(the oracle's restatement of the reference CPU path, the whole job on the
+host cores) and the serving stream (Poisson arrivals over 256^3 / 512^3 /
+1024^3 requests, each with its own seeded phantom, FP16 storage, 200 GB HBM
+budget): p50 / p99 latency and aggregate GUPS.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
 """
 import argparse
 import json
-import math
 import os
 import statistics
 import sys
@@ -34,22 +48,25 @@ sys.path.insert(0, ROOT)
 import __graft_entry__  # noqa: E402
 
 HBM_PEAK_GBPS = 8000.0
+# MI355X published FP32 vector peak 157.3 TFLOP/s = 2 FLOP x 64 lanes x
+# this many wave64 VALU instructions per second (packed FP32 counted as two)
+VALU_PEAK_GINSTR = 157.3e3 / 2 / 64
 WORKLOAD = "fdk_512"
 FFT_LENGTH = 2048
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "backproject_fdk512_traffic.md")
+PROFILES = os.path.join(ROOT, "profiles")
 
 
-def read_traffic(key):
-    """HBM bytes per backprojection launch measured with rocprofv3 FETCH_SIZE
-    and WRITE_SIZE (each scaled by the same counter's reading for a plain copy
-    of known size), as recorded in the profile summary: a line `key = N`."""
-    if not os.path.exists(TRAFFIC_FILE):
+def profile_value(name, key):
+    """A `key = value` line of a profiles/ summary (rocprofv3 counter
+    figures the bench does not measure itself), or None."""
+    path = os.path.join(PROFILES, name)
+    if not os.path.exists(path):
         return None
-    with open(TRAFFIC_FILE) as fh:
+    with open(path) as fh:
         for line in fh:
             parts = line.strip().strip("`").split("=")
             if len(parts) == 2 and parts[0].strip() == key:
-                return int(parts[1].strip())
+                return float(parts[1].strip())
     return None
 
 
@@ -58,18 +75,25 @@ def log(msg):
 
 
 def make_inputs(ctr, g, dev, seed=20240602):
+    """Phantom -> forward projection (timed: the forward-projector line) ->
+    Poisson noise and log."""
     import torch
     ell = ctr.random_ellipsoids(seed, 30, (g.X, g.Y, g.Z), g.voxel, (5, 120))
     ell[:, 9] *= 0.02  # density 1.0 = 0.02 / mm
     vol = torch.zeros((g.Z, g.Y, g.X), dtype=torch.float32, device=dev)
     ctr.render_ellipsoids(ell, vol, g.volume_width)
-    t0 = time.time()
-    proj = ctr.forward_project(vol, g, step=1.0)
+    proj = ctr.forward_project(vol, g, step=1.0)  # warm-up
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    ctr.forward_project(vol, g, step=1.0, out=proj)
+    e1.record()
+    e1.synchronize()
+    fwd_ms = e0.elapsed_time(e1)
     ctr.poisson_log(proj, 1e5, seed)
     torch.cuda.synchronize()
-    log(f"inputs: phantom + forward projection + noise in {time.time() - t0:.1f} s")
+    log(f"inputs ready; forward projection {fwd_ms:.1f} ms")
     del vol
-    return proj
+    return proj, fwd_ms
 
 
 def time_fdk(ctr, g, proj, dtype, steps, warmup, ws, scale):
@@ -96,88 +120,87 @@ def time_fdk(ctr, g, proj, dtype, steps, warmup, ws, scale):
     return step_ms, bp_ms, vol
 
 
-def cpu_baseline(g, proj_cpu, scale, threads, budget_s=15.0):
-    """The oracle's FP32 voxel loop + reference-algorithm row filter on a
-    bounded sample of the same workload, extrapolated to the full job."""
+def cpu_baseline(g, proj_cpu, scale, threads, full=True, budget_s=6.0):
+    """The reference's CPU path as the oracle restates it: the row filter
+    with the reference's radix-2 FFT algorithm (FP64, bit-exact with its
+    compiled FFT) and the FP32 voxel-driven cone backprojection (FP64
+    geometry).  full=True runs the whole job; otherwise a bounded sample of
+    rows and z-slices, scaled to the job."""
     import oracle
     n_rows = g.nv * g.n_views
-    sample_rows = min(n_rows, 4096)
-    rows = proj_cpu.reshape(-1, g.nu)[:sample_rows]
+    rows = proj_cpu.reshape(-1, g.nu)
+    sample_rows = n_rows if full else min(n_rows, 2048)
     t0 = time.perf_counter()
-    oracle.filter_rows(rows, oracle.RAM_LAK, FFT_LENGTH, threads=threads)
+    q = oracle.filter_rows(rows[:sample_rows], oracle.RAM_LAK, FFT_LENGTH, threads=threads)
     t_filter = (time.perf_counter() - t0) * n_rows / sample_rows
-    q = proj_cpu  # filter output values do not change the voxel loop's cost
-    slices = 1
-    while True:
+    if full:
+        q = q.astype(np.float32).reshape(proj_cpu.shape)
         t0 = time.perf_counter()
         oracle.backproject_cone_f32(q, g.source_distance, g.detector_distance,
                                     g.detector_width, g.volume_width, 1.0, 0.0, scale,
-                                    (g.X, g.Y, g.Z), (g.Z // 2, g.Z // 2 + slices),
-                                    threads=threads)
-        dt = time.perf_counter() - t0
-        if dt > budget_s / 3 or slices >= g.Z:
-            break
-        slices = min(g.Z, max(slices * 2, int(slices * budget_s / 3 / max(dt, 1e-3))))
-    t_bp = dt * g.Z / slices
+                                    (g.X, g.Y, g.Z), threads=threads)
+        t_bp = time.perf_counter() - t0
+        sample = (f"full job: all {n_rows} detector rows filtered (FFT {FFT_LENGTH}) and all "
+                  f"{g.Z} z-slices x {g.n_views} views backprojected (filter {t_filter:.1f} s, "
+                  f"backprojection {t_bp:.1f} s)")
+    else:
+        slices = 1
+        while True:
+            t0 = time.perf_counter()
+            oracle.backproject_cone_f32(proj_cpu, g.source_distance, g.detector_distance,
+                                        g.detector_width, g.volume_width, 1.0, 0.0, scale,
+                                        (g.X, g.Y, g.Z), (g.Z // 2, g.Z // 2 + slices),
+                                        threads=threads)
+            dt = time.perf_counter() - t0
+            if dt > budget_s / 3 or slices >= g.Z:
+                break
+            slices = min(g.Z, max(slices * 2, int(slices * budget_s / 3 / max(dt, 1e-3))))
+        t_bp = dt * g.Z / slices
+        sample = (f"sample scaled to the job: {sample_rows} of {n_rows} rows filtered and "
+                  f"{slices} of {g.Z} z-slices backprojected (filter {t_filter:.1f} s, "
+                  f"backprojection {t_bp:.1f} s extrapolated)")
     total = t_filter + t_bp
-    return {
-        "value": g.updates / total / 1e9,
-        "unit": "GUPS",
-        "cores": threads,
-        "kind": "port",
-        "sample": (f"oracle filter of {sample_rows} of {n_rows} detector rows "
-                   f"(FFT {FFT_LENGTH}, reference radix-2 algorithm, FP64) + FP32 "
-                   f"backprojection of {slices} of {g.Z} z-slices x {g.n_views} views; "
-                   f"times scaled to the full job (filter {t_filter:.1f} s, "
-                   f"backprojection {t_bp:.1f} s)"),
-    }
+    return {"value": g.updates / total / 1e9, "unit": "GUPS", "cores": threads,
+            "kind": "port", "seconds": round(total, 2), "sample": sample}
 
 
-def serving_main(args):
+def run_serving(ctr, requests, rate, budget_gb, pack, max_slots):
     """Poisson stream of FDK requests over three sizes (256^3/360/384^2,
-    512^3/720/768^2, 1024^3/1200/1024^2), FP16 filtered projections, FP32
-    volumes, 200 GB HBM budget: p50/p99 latency and aggregate GUPS."""
+    512^3/720/768^2, 1024^3/1200/1024^2), each with its own seeded phantom,
+    FP16 filtered projections, FP32 volumes: p50/p99 latency and aggregate
+    GUPS."""
     import torch
-    ctr = __graft_entry__.load()
-    ctr.require_device()
     from ctrecon import serving
-    small, medium, large = (int(x) for x in args.max_slots.split(","))
-    srv = serving.Server(budget_bytes=args.budget_gb * 1e9, dtype=ctr.F16, pack=args.pack,
-                         max_slots={"small": small, "medium": medium, "large": large},
-                         pool=2, log=log)
+    small, medium, large = (int(x) for x in max_slots.split(","))
+    srv = serving.Server(budget_bytes=budget_gb * 1e9, dtype=ctr.F16, pack=pack,
+                         max_slots={"small": small, "medium": medium, "large": large}, log=log)
     classes = ["small", "medium", "large"]
-    t0 = time.time()
-    srv.prepare_sources(classes)
-    log(f"inputs ready in {time.time() - t0:.1f} s")
     srv.warm(classes)
     torch.cuda.synchronize()
-    reqs = serving.poisson_arrivals(args.requests, args.rate, seed=7)
-    log(f"serving {args.requests} requests at {args.rate}/s")
+    reqs = serving.poisson_arrivals(requests, rate, seed=7)
+    log(f"serving {requests} requests at {rate}/s")
     makespan = srv.run(reqs)
     s = serving.summarize(reqs, makespan, srv.geoms)
-    out = {
-        "metric": "FDK serving latency (Poisson arrivals, sizes 256^3/512^3/1024^3, FP16 store)",
-        "value": round(s["p99_ms"], 2),
-        "unit": "ms (p99 latency)",
-        "n_gpus": 1,
-        "higher_is_better": False,
+    return {
+        "metric": "FDK serving latency, Poisson arrivals",
+        "rate_rps": rate,
+        "requests": requests,
         "latency_ms": {"p50": round(s["p50_ms"], 2), "p99": round(s["p99_ms"], 2),
                        "mean": round(s["mean_ms"], 2)},
         "aggregate_gups": round(s["aggregate_gups"], 2),
         "throughput_rps": round(s["throughput_rps"], 3),
-        "rate_rps": args.rate,
-        "requests": args.requests,
         "makespan_s": round(makespan, 2),
-        "hbm_budget_gb": args.budget_gb,
-        "max_slots": args.max_slots,
+        "per_class": {c: {k: round(v, 2) if isinstance(v, float) else v for k, v in d.items()}
+                      for c, d in s["per_class"].items()},
+        "hbm_budget_gb": budget_gb,
         "slots": {c: sum(1 for x in srv.slots if x.cls == c) for c in classes},
         "slot_gb": round(srv.used / 1e9, 2),
-        "per_class": s["per_class"],
-        "dtype": "f32",
-        "data": "synthetic: seeded ellipsoid phantoms, ray-driven projections, Poisson I0=1e5 "
-                "(2 phantoms per size class, reused by request seed)",
+        "pack": pack,
+        "config": "sizes uniform over 256^3/360 views/384^2, 512^3/720/768^2, "
+                  "1024^3/1200/1024^2; FP16 store, FP32 volume, FFT 2 nu",
+        "data": "each request its own seeded ellipsoid phantom; its exact projections are "
+                "generated on the device at admission (counted in its latency)",
     }
-    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -186,10 +209,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--mode", choices=["fdk", "serving", "copy-probe"], default="fdk")
     ap.add_argument("--requests", type=int, default=1000)
-    ap.add_argument("--rate", type=float, default=3.0)
+    ap.add_argument("--rate", type=float, default=4.0)
     ap.add_argument("--pack", type=int, default=4)
     ap.add_argument("--budget-gb", type=float, default=200.0)
     # one 1024^3 slot: a 1024^3 reconstruction fills the GPU for ~0.35 s, and
@@ -202,15 +225,10 @@ def main():
               "(the reconstruction would shard into z-slabs with no exchange; "
               "a slab is the same kernel with fewer y-runs)", file=sys.stderr)
         sys.exit(2)
-    if args.mode == "serving":
-        serving_main(args)
-        return
     if args.mode == "copy-probe":
         # a plain elementwise kernel that reads 1 GiB and writes 1 GiB, the
         # reference for scaling the HBM byte counters (FETCH_SIZE /
-        # WRITE_SIZE) of the profiled kernels.  (A clone goes through the
-        # runtime's copy path instead, whose counter readings for the same
-        # 1 GiB varied between runs by a quarter.)
+        # WRITE_SIZE) of the profiled kernels
         import torch
         x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
         y = torch.empty_like(x)
@@ -223,6 +241,11 @@ def main():
     import torch
     ctr = __graft_entry__.load()
     ctr.require_device()
+    if args.mode == "serving":
+        print(json.dumps(run_serving(ctr, args.requests, args.rate, args.budget_gb, args.pack,
+                                     args.max_slots)), flush=True)
+        return
+
     dev = torch.device("cuda:0")
     g = ctr.config_geometry(WORKLOAD)
     scale = g.fbp_scale(FFT_LENGTH)
@@ -230,7 +253,7 @@ def main():
     ws.reserve(g.n_views, FFT_LENGTH)
     log(f"workload {WORKLOAD}: {g.X}x{g.Y}x{g.Z} from {g.n_views} views of "
         f"{g.nu}x{g.nv}, FFT {FFT_LENGTH}")
-    proj = make_inputs(ctr, g, dev)
+    proj, fwd_ms = make_inputs(ctr, g, dev)
     ws.set_interpolation(ctr.INTERP_FP32)
 
     log("timing FP32-stored form")
@@ -240,8 +263,8 @@ def main():
     log("timing FP16-stored form")
     f16_ms, f16_bp, vol16 = time_fdk(ctr, g, proj, ctr.F16, args.steps, args.warmup, ws, scale)
     log(f"  step {statistics.median(f16_ms):.2f} ms, backprojection {statistics.median(f16_bp):.2f} ms")
-    diff = (vol16.double() - ref32.double()).norm() / ref32.double().norm()
-    rel_rmse_16 = float(diff)
+    rel_rmse_16 = float((vol16.double() - ref32.double()).norm() / ref32.double().norm())
+    del vol16
     log("timing BF16-stored form")
     b16_ms, b16_bp, volb = time_fdk(ctr, g, proj, ctr.BF16, max(3, args.steps // 2), 1, ws, scale)
     rel_rmse_bf = float((volb.double() - ref32.double()).norm() / ref32.double().norm())
@@ -259,33 +282,65 @@ def main():
                                       / ref32.double().norm())}
         del v
     ws.set_interpolation(ctr.INTERP_FP32)
-
+    del ref32, vol32
     torch.cuda.synchronize()
+
     ms = statistics.mean(f16_ms)
     gups = g.updates / (ms * 1e-3) / 1e9
     gups32 = g.updates / (statistics.mean(f32_ms) * 1e-3) / 1e9
     bp_ms = statistics.mean(f16_bp)
     bp_ms32 = statistics.mean(f32_bp)
-    b_alg16 = (4 * 2 + 8.0 / g.n_views) * g.updates
-    b_alg32 = (4 * 4 + 8.0 / g.n_views) * g.updates
-    ach16 = b_alg16 / (bp_ms * 1e-3) / 1e9
-    ach32 = b_alg32 / (bp_ms32 * 1e-3) / 1e9
-    traffic = read_traffic("fp16_bytes_per_launch")
+
+    def valu(form, t_ms):
+        per_update = profile_value("backproject_fdk512_sq_counters.md",
+                                   f"{form}_valu_per_update")
+        if per_update is None:
+            return None, None
+        ginstr = per_update * g.updates / 64 / (t_ms * 1e-3) / 1e9
+        return per_update, ginstr
+
+    v16, achieved16 = valu("fp16", bp_ms)
+    v32, achieved32 = valu("fp32", bp_ms32)
+    traffic16 = profile_value("backproject_fdk512_traffic.md", "fp16_bytes_per_launch")
+    traffic32 = profile_value("backproject_fdk512_traffic.md", "fp32_bytes_per_launch")
+    tap_bytes16 = (4 * 2 + 8.0 / g.n_views) * g.updates
+    compulsory = ctr.staged_layout(g.nu, g.nv, g.n_views)[2] + 4.0 * g.X * g.Y * g.Z
+
+    # forward projector on the same scan: rays and trilinear samples per
+    # second; bytes from beyond L2 against reading the volume once per view
+    rays = g.nu * g.nv * g.n_views
+    fwd_beyond_l2 = profile_value("forward_fdk512_counters.md", "slab_beyond_l2_bytes")
+    forward = {"kernel": "forward_slab_kernel (step 1 voxel)", "ms": round(fwd_ms, 2),
+               "grays_per_s": round(rays / (fwd_ms * 1e-3) / 1e9, 3),
+               "gsamples_per_s": round(rays * g.X / (fwd_ms * 1e-3) / 1e9, 1),
+               "beyond_l2_bytes": fwd_beyond_l2,
+               "volume_once_per_view_bytes": 4.0 * g.X * g.Y * g.Z * g.n_views,
+               "beyond_l2_gbps": (round(fwd_beyond_l2 / (fwd_ms * 1e-3) / 1e9, 1)
+                                  if fwd_beyond_l2 else None)}
 
     cpu = None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        log(f"CPU baseline on {threads} threads")
         proj_cpu = proj.cpu().numpy()
-        cpu = cpu_baseline(g, proj_cpu, scale, threads, args.cpu_budget)
-        log(f"  {cpu['value']:.3f} GUPS")
+        log(f"CPU baseline, whole job on {threads} threads")
+        cpu = cpu_baseline(g, proj_cpu, scale, threads, full=True)
+        log(f"  {cpu['value']:.3f} GUPS ({cpu['seconds']} s)")
         if threads > 1:
-            log("CPU baseline on 1 thread")
-            one = cpu_baseline(g, proj_cpu, scale, 1, min(6.0, args.cpu_budget / 2))
+            log("CPU baseline on 1 thread (sampled: the whole job would take ~7 min)")
+            one = cpu_baseline(g, proj_cpu, scale, 1, full=False)
             cpu["single_core"] = {"value": one["value"], "unit": "GUPS", "cores": 1,
                                   "sample": one["sample"]}
             log(f"  {one['value']:.4f} GUPS")
         del proj_cpu
+    del proj
+    torch.cuda.empty_cache()
+
+    serving_line = None
+    if not args.no_serving:
+        serving_line = run_serving(ctr, args.requests, args.rate, args.budget_gb, args.pack,
+                                   args.max_slots)
+        log(f"serving p50 {serving_line['latency_ms']['p50']} ms, "
+            f"p99 {serving_line['latency_ms']['p99']} ms")
 
     out = {
         "metric": "FDK GUPS 512^3 x 720 views (FP16-stored filtered projections, FP32 "
@@ -308,23 +363,39 @@ def main():
                        "p99": round(float(np.percentile(f16_ms, 99)), 3)},
         "fp32_store": {"gups": round(gups32, 3), "ms_per_step": round(statistics.mean(f32_ms), 3),
                        "backproject_ms": round(bp_ms32, 3),
-                       "roofline_frac": round(ach32 / HBM_PEAK_GBPS, 4),
-                       "traffic": read_traffic("fp32_bytes_per_launch")},
+                       "valu_ginstr_per_s": round(achieved32, 1) if achieved32 else None,
+                       "valu_frac": round(achieved32 / VALU_PEAK_GINSTR, 4) if achieved32 else None,
+                       "hbm_bytes_per_launch": traffic32,
+                       "hbm_frac": (round(traffic32 / (bp_ms32 * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
+                                    if traffic32 else None)},
         "fp16_vs_fp32_gups": round(gups / gups32, 3),
         "rel_rmse_fp16_vs_fp32": rel_rmse_16,
         "bf16_store": {"gups": round(g.updates / (statistics.mean(b16_ms) * 1e-3) / 1e9, 3),
                        "rel_rmse_vs_fp32": rel_rmse_bf},
         "interp_16bit_arithmetic": fast,
         "backproject_gups_fp16": round(g.updates / (bp_ms * 1e-3) / 1e9, 3),
-        "roofline": {"bound": "hbm", "achieved": round(ach16, 1), "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": round(ach16 / HBM_PEAK_GBPS, 4),
-                     "traffic": traffic,
-                     "traffic_frac": (round(traffic / (bp_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
-                                      if traffic else None),
-                     "kernel": "backproject_lds_kernel<fp16, 32> (FP32 interpolation; LDS 48 "
-                               "columns, pitch 51, 72 rows)",
-                     "avg_ms": round(bp_ms, 3)},
+        "roofline": {
+            "bound": "valu",
+            "achieved": round(achieved16, 1) if achieved16 else None,
+            "peak": round(VALU_PEAK_GINSTR, 1),
+            "unit": "G wave64 VALU instructions/s",
+            "frac": round(achieved16 / VALU_PEAK_GINSTR, 4) if achieved16 else None,
+            "frac_of_unpacked_issue": (round(2 * achieved16 / VALU_PEAK_GINSTR, 4)
+                                       if achieved16 else None),
+            "valu_per_update": v16,
+            "kernel": "backproject_lds_kernel<fp16, 32> (FP32 interpolation; LDS 48 columns, "
+                      "pitch 51, 72 rows)",
+            "avg_ms": round(bp_ms, 3),
+            "traffic": traffic16,
+            "hbm_gbps": round(traffic16 / (bp_ms * 1e-3) / 1e9, 1) if traffic16 else None,
+            "hbm_frac": (round(traffic16 / (bp_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
+                         if traffic16 else None),
+            "compulsory_bytes": compulsory,
+            "lds_served_tap_bytes": tap_bytes16,
+        },
+        "forward_projector": forward,
         "cpu_baseline": cpu,
+        "serving": serving_line,
     }
     print(json.dumps(out), flush=True)
 

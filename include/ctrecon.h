@@ -333,6 +333,19 @@ int ctr_rebin_fan_to_parallel(void *stream, const float *fan, int nu_fan,
 int ctr_render_ellipsoids(void *stream, ctr_workspace *ws,
                           const double *ellipsoids, int count, float *volume,
                           int X, int Y, int Z, double volume_width);
+/* Exact line integrals of ellipsoids (records as in ctr_render_ellipsoids)
+ * along the rays of ctr_forward_project: p = sum_e density_e * chord_e,
+ * written to (accumulate = 0) or added to (accumulate != 0) `out` in the
+ * given layout.  The chords are FP64; each ellipsoid is first culled per
+ * ray against its bounding sphere in FP32 with a safe margin.  Makes the
+ * seeded input of each serving request without a voxelised phantom.  No
+ * reference counterpart (the reference projects voxel volumes,
+ * FBP3D.cpp:269-330). */
+int ctr_project_ellipsoids(void *stream, int geometry, const double *ellipsoids, int count,
+                           double source_distance, double detector_distance,
+                           double detector_width, double rotation_count,
+                           double rotation_offset, float *out, int layout, int nu, int nv,
+                           int n_views, int accumulate);
 /* Replaces every line integral p by -ln(max(K, 1) / i0) with
  * K ~ Poisson(i0 * exp(-p)) from a counter-based generator keyed by
  * (seed, element index): the noise model of the benchmark's cone config.

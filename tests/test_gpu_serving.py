@@ -4,12 +4,18 @@
   ctr_backproject gives for that job alone (mixed sizes and geometries).
 * A request reconstructed alone and the same request served in a mix of
   concurrently admitted requests (different classes, packed small ones,
-  several slots and streams in flight) give the same volume.
+  several slots and streams in flight) give the same volume -- on toy sizes
+  and at the serving workload's real sizes (1024^3 from 1200 views of 1024^2
+  mixed with packed 256^3 and a 512^3), where one served volume is also
+  checked against the FP64 oracle on a crop.
+* Each request's projections are its own seeded phantom's exact line
+  integrals (ctr_project_ellipsoids, checked against the oracle's analytic
+  projector).
 """
 import numpy as np
 import pytest
 
-from conftest import rel_max
+from conftest import rel_max, rel_rmse
 
 pytestmark = pytest.mark.gpu
 
@@ -53,14 +59,13 @@ def test_alone_equals_mixed(ctr, dev):
     import torch
     from ctrecon import serving
     geoms = _geoms(ctr)
-    mk = lambda: serving.Server(budget_bytes=4e9, pack=3, keep_results=True, pool=3,
+    mk = lambda: serving.Server(budget_bytes=4e9, pack=3, keep_results=True,
                                 geometries=geoms,
                                 max_slots={"small": 2, "medium": 2, "large": 2})
     mix = [serving.Request(i, c, s, 0.0) for i, (c, s) in enumerate(
         [("large", 0), ("small", 1), ("small", 2), ("medium", 0), ("small", 0),
          ("large", 1), ("small", 1), ("medium", 2), ("small", 2), ("medium", 1)])]
     srv = mk()
-    srv.prepare_sources(["small", "medium", "large"])
     srv.warm(["small", "medium", "large"])
     makespan = srv.run(mix)
     assert all(np.isfinite(r.latency) and r.latency >= 0 for r in mix)
@@ -78,8 +83,88 @@ def test_alone_equals_mixed(ctr, dev):
 def test_server_respects_budget(ctr, dev):
     from ctrecon import serving
     geoms = _geoms(ctr)
-    srv = serving.Server(budget_bytes=0.05e9, pack=2, pool=1, geometries=geoms)
+    srv = serving.Server(budget_bytes=0.05e9, pack=2, geometries=geoms)
     est = {c: srv._estimate(c) for c in geoms}
     srv.warm(["large", "small"])
     assert srv.used <= 0.05e9
     assert all(s.bytes <= est[s.cls] for s in srv.slots)
+
+
+@pytest.mark.parametrize("kind", ["cone", "parallel"])
+def test_project_ellipsoids_vs_oracle(ctr, orc, dev, kind):
+    """The analytic projector against the oracle's FP64 line integrals (both
+    layouts, more ellipsoids than one launch carries, accumulate)."""
+    import torch
+    ell = ctr.random_ellipsoids(8, 70, (64, 48, 64), 1.0, (2, 25))
+    if kind == "cone":
+        g = ctr.Geometry(ctr.CONE, nu=96, nv=72, n_views=20, X=64, Y=48, Z=64,
+                         detector_width=200.0, volume_width=64.0, source_distance=150.0,
+                         detector_distance=120.0, rotation_offset=0.3)
+        ref = orc.analytic_cone(ell, g.source_distance, g.detector_distance, g.detector_width,
+                                g.nu, g.nv, g.n_views, 1.0, 0.3)
+    else:
+        g = ctr.Geometry(ctr.PARALLEL, nu=96, nv=72, n_views=20, X=64, Y=48, Z=64,
+                         detector_width=100.0, volume_width=64.0, rotation_count=0.5,
+                         rotation_offset=0.3)
+        ref = orc.analytic_parallel(ell, g.detector_width, g.nu, g.nv, g.n_views, 0.5, 0.3)
+    got = ctr.project_ellipsoids(ell, g).cpu().numpy()
+    assert ref.max() > 1.0
+    # FP64 chords, rounded to FP32 once
+    assert np.abs(got - ref).max() <= 2e-7 * np.abs(ref).max()
+    sino = ctr.project_ellipsoids(ell, g, layout=ctr.SINOGRAMS).cpu().numpy()
+    assert np.array_equal(sino.transpose(1, 0, 2), got)
+    twice = ctr.project_ellipsoids(ell, g, out=torch.from_numpy(got.copy()).to(dev),
+                                   accumulate=True).cpu().numpy()
+    assert np.allclose(twice, 2 * got, rtol=1e-6, atol=0)
+
+
+def test_serving_at_real_sizes(ctr, orc, dev):
+    """The serving workload's classes at their real sizes: one 1024^3 request
+    (1200 views of 1024^2) admitted with five 256^3 requests (360 views of
+    384^2; packed three to a launch) and one 512^3 (720 views of 768^2), FP16
+    storage.  Every result equals the same request served alone, and the
+    1024^3 volume matches the FP64 oracle pipeline (FDK weight, reference
+    ramp filter, voxel loop) of the same projections on a 16^3 crop."""
+    import torch
+    from ctrecon import serving
+    mix = [serving.Request(i, c, s, 0.0) for i, (c, s) in enumerate(
+        [("large", 11), ("small", 1), ("small", 2), ("medium", 3), ("small", 4),
+         ("small", 5), ("small", 6)])]
+    srv = serving.Server(budget_bytes=200e9, pack=3, keep_results=True,
+                         max_slots={"small": 2, "medium": 1, "large": 1})
+    srv.warm(["small", "medium", "large"])
+    print("[c4] slots warm", flush=True)
+    srv.run(mix)
+    print("[c4] mix served", flush=True)
+    for r in mix:
+        alone = serving.Request(100 + r.rid, r.cls, r.seed, 0.0)
+        srv.run([alone])
+        assert torch.equal(alone.result, r.result), (r.cls, r.seed)
+        assert alone.checksum == r.checksum
+    print("[c4] alone == mixed", flush=True)
+    large = mix[0]
+    g = srv.geoms["large"]
+    assert (g.X, g.n_views, g.nu) == (1024, 1200, 1024)
+    box = ((504, 520), (504, 520), (500, 516))
+    # the crop projects within +-12 detector rows of the centre: the oracle
+    # needs only a band of rows, which the FDK weight and the row filter
+    # treat exactly as the full panel (same pitch, same centre)
+    h = 64
+    r0 = g.nv // 2 - h // 2
+    proj = serving.make_projections(g, large.seed, device=dev)
+    band = proj[:, r0:r0 + h, :].cpu().numpy()
+    del proj
+    w = orc.fdk_weight(band, g.source_distance, g.detector_distance, g.detector_width)
+    slot = next(s for s in srv.slots if s.cls == "large")
+    q = orc.filter_rows(w.reshape(-1, g.nu), orc.RAM_LAK, slot.fft).reshape(w.shape)
+    ref = orc.backproject_cone_region(q.astype(np.float32), g.source_distance,
+                                      g.detector_distance, g.detector_width, g.volume_width,
+                                      1.0, 0.0, slot.scale, (g.X, g.Y, g.Z), box)
+    ref = np.maximum(ref, 0.0)  # the served pipeline clips negatives
+    (i0, i1), (j0, j1), (k0, k1) = box
+    got = large.result[k0:k1, j0:j1, i0:i1].cpu().numpy()
+    err = rel_rmse(got, ref)
+    print(f"[c4] 1024^3 crop vs oracle: rel RMSE {err:.2e}, peak {np.abs(ref).max():.3e}",
+          flush=True)
+    assert np.abs(ref).max() > 0
+    assert err < 1e-3
