@@ -79,6 +79,8 @@ SIGNATURES = {
     "ctr_stage_projections": [_v, _v, _v, _i, _i, _i, _i, _v, _i, _v],
     "ctr_backproject": [_v, _v, _i, _v, _i, _v, _i, _i, _i, _d, _d, _d, _d, _d,
                         _d, _v, _i, _i, _i, _f, _i, _i],
+    "ctr_backproject_noalloc": [_v, _i, _v, _i, _v, _i, _i, _i, _d, _d, _d, _d, _d, _d,
+                                _v, _i, _i, _i, _f, _i, _i],
     "ctr_forward_project": [_v, _v, _i, _v, _i, _i, _i, _d, _d, _d, _d, _d, _d,
                             _v, _i, _i, _i, _i, _d],
     "ctr_fdk_weight": [_v, _v, _i, _i, _i, _i, _d, _d, _d],

@@ -19,7 +19,7 @@
 //
 // A 256-thread block is 4 wave64s, each covering an 8 x 8 patch of columns,
 // so the 64 lanes of a wave sample a compact detector footprint.
-#include <atomic>
+#include <mutex>
 #include <cstdlib>
 #include <cstring>
 
@@ -1067,10 +1067,9 @@ static BpJob make_job(const void *staged, const StagedLayout &L, int nu, int nv,
 // Kernel choice: the LDS-staged kernel for volumes at least 32 voxels along
 // the rotation axis, the direct-tap kernel otherwise; the workspace's
 // CTR_OPTION_BACKPROJECT_KERNEL overrides (diagnostics and A/B timing).
-static bool use_lds_kernel(const ctr_workspace *ws, int Y)
+static bool use_lds_kernel(int kernel_option, int Y)
 {
-    const int ov = workspace_option(ws, CTR_OPTION_BACKPROJECT_KERNEL);
-    return ov == CTR_KERNEL_LDS || (ov == CTR_KERNEL_AUTO && Y >= 32);
+    return kernel_option == CTR_KERNEL_LDS || (kernel_option == CTR_KERNEL_AUTO && Y >= 32);
 }
 
 // Fills the footprint table of job J at `dst` (gx * gy * n_views float4).
@@ -1119,9 +1118,9 @@ static void launch_lds_packed(hipStream_t s, BpBatch B, unsigned blocks)
 // LDS tile variant of a staged format under the workspace's interpolation
 // arithmetic: the plain format (FP32 interpolation) or the kF16Packed /
 // kBF16Dot variants (CTR_INTERP_FAST).
-static int lds_variant(const ctr_workspace *ws, int staged_dtype, bool f16_weights_ok)
+static int lds_variant(int interpolation, int staged_dtype, bool f16_weights_ok)
 {
-    if (workspace_option(ws, CTR_OPTION_INTERPOLATION) == CTR_INTERP_FP32) return staged_dtype;
+    if (interpolation == CTR_INTERP_FP32) return staged_dtype;
     if (staged_dtype == CTR_F16) return f16_weights_ok ? kF16Packed : CTR_F16;
     if (staged_dtype == CTR_BF16) return kBF16Dot;
     return staged_dtype;
@@ -1161,6 +1160,87 @@ static int launch_bp(hipStream_t s, const BpJob &J, bool lds, int variant)
     return CTR_OK;
 }
 
+// Host-side checks shared by ctr_backproject and ctr_backproject_noalloc.
+static int check_backproject(int geometry, const void *staged, int staged_dtype,
+                             const float *view_scale, int nu, int nv, int n_views, double sod,
+                             double odd, double detector_width, double volume_width,
+                             const float *volume, int X, int Y, int Z)
+{
+    CTR_REQUIRE(staged && volume, CTR_ERR_NULL_POINTER,
+                "[CTAnalyzer Error] ctr_backproject: projections or volume is NULL");
+    CTR_REQUIRE(staged_dtype == CTR_F32 || staged_dtype == CTR_F16 || staged_dtype == CTR_BF16,
+                CTR_ERR_INVALID_ARGUMENT, "unknown staged dtype %d", staged_dtype);
+    CTR_REQUIRE(staged_dtype == CTR_F32 || view_scale, CTR_ERR_NULL_POINTER,
+                "view_scale is required for 16-bit staged projections");
+    int rc = check_geometry(geometry, sod, odd, detector_width, volume_width, X, Y, Z, nu, nv,
+                            n_views);
+    if (rc) return rc;
+    StagedLayout L = staged_layout(nu, nv, n_views, staged_dtype);
+    CTR_REQUIRE(L.view * 4 < kMaxViewBytes, CTR_ERR_INVALID_ARGUMENT,
+                "staged view of %zu bytes exceeds the backprojector's 32-bit in-view "
+                "offsets (%zu bytes)", L.view * 4, kMaxViewBytes);
+    return CTR_OK;
+}
+
+// One backprojection with its scratch given: the per-view geometry records
+// at `geom_buf` (kGeomBytesPerView per view) and, when the LDS kernel runs,
+// the tile footprint table at `foot_buf` (footprint_bytes).  Both tables are
+// written whole before they are read, so no scratch needs clearing.
+static int backproject_with(hipStream_t s, int kernel_option, int interpolation, int geometry,
+                            const void *staged, int staged_dtype, const float *view_scale,
+                            int nu, int nv, int n_views, double sod, double odd,
+                            double detector_width, double volume_width, double rotation_count,
+                            double rotation_offset, float *volume, int X, int Y, int Z,
+                            float scale, int accumulate, int clip_negative, void *geom_buf,
+                            void *foot_buf)
+{
+    GeomPtrs g;
+    int rc = build_geometry_at(s, geom_buf, geometry, n_views, sod, odd, detector_width,
+                               volume_width, rotation_count, rotation_offset, X, Y, Z, nu, nv, &g);
+    if (rc) return rc;
+    StagedLayout L = staged_layout(nu, nv, n_views, staged_dtype);
+    BpJob J = make_job(staged, L, nu, nv, g, view_scale, n_views, volume, X, Y, Z, scale,
+                       accumulate, clip_negative);
+    const bool lds = use_lds_kernel(kernel_option, Y);
+    if (lds && (rc = build_footprints(s, J, (float4 *)foot_buf))) return rc;
+    const int variant =
+        lds_variant(interpolation, staged_dtype, f16_weights_fit(J, geometry, sod, volume_width));
+    switch (staged_dtype) {
+    case CTR_F32: return launch_bp<CTR_F32>(s, J, lds, variant);
+    case CTR_F16: return launch_bp<CTR_F16>(s, J, lds, variant);
+    default: return launch_bp<CTR_BF16>(s, J, lds, variant);
+    }
+}
+
+// Scratch of ctr_backproject_noalloc: the geometry and footprint tables in a
+// device array of the library itself (each GPU holds its own copy for the
+// life of the process), so that entry point allocates nothing.  Host code
+// uses the per-device address from hipGetSymbolAddress, never the array's
+// name.  Every ctr_backproject_noalloc call on a GPU shares its array:
+// calls that overlap on one GPU race.
+constexpr size_t kNoAllocScratchBytes = (size_t)128 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static std::mutex g_scratch_lock;
+static void *g_scratch_ptr[64] = {nullptr};
+
+// The calling device's address of `scratch`, looked up once per device.
+static int scratch_address(void **out)
+{
+    int dev = 0;
+    CTR_TRY_HIP(hipGetDevice(&dev));
+    CTR_REQUIRE(dev >= 0 && dev < 64, CTR_ERR_INVALID_ARGUMENT, "device index %d out of range",
+                dev);
+    std::lock_guard<std::mutex> g(g_scratch_lock);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        CTR_TRY_HIP(hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)));
+        CTR_REQUIRE(ptr, CTR_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return CTR_OK;
+}
+
 }  // namespace ctr
 
 using namespace ctr;
@@ -1175,45 +1255,54 @@ int ctr_backproject(void *stream, ctr_workspace *ws, int geometry, const void *s
                     int clip_negative)
 {
     clear_error();
-    CTR_REQUIRE(staged && volume, CTR_ERR_NULL_POINTER,
-                "[CTAnalyzer Error] ctr_backproject: projections or volume is NULL");
-    CTR_REQUIRE(staged_dtype == CTR_F32 || staged_dtype == CTR_F16 || staged_dtype == CTR_BF16,
-                CTR_ERR_INVALID_ARGUMENT, "unknown staged dtype %d", staged_dtype);
-    CTR_REQUIRE(staged_dtype == CTR_F32 || view_scale, CTR_ERR_NULL_POINTER,
-                "view_scale is required for 16-bit staged projections");
-    int rc = check_geometry(geometry, source_distance, detector_distance, detector_width,
-                            volume_width, X, Y, Z, nu, nv, n_views);
+    int rc = check_backproject(geometry, staged, staged_dtype, view_scale, nu, nv, n_views,
+                               source_distance, detector_distance, detector_width, volume_width,
+                               volume, X, Y, Z);
     if (rc) return rc;
-    StagedLayout L = staged_layout(nu, nv, n_views, staged_dtype);
-    CTR_REQUIRE(L.view * 4 < kMaxViewBytes, CTR_ERR_INVALID_ARGUMENT,
-                "staged view of %zu bytes exceeds the backprojector's 32-bit in-view "
-                "offsets (%zu bytes)", L.view * 4, kMaxViewBytes);
     ctr_workspace *w = nullptr;
     if ((rc = workspace_resolve(ws, &w))) return rc;
-    hipStream_t s = as_stream(stream);
-    void *p = nullptr;
-    if ((rc = workspace_get(w, SLOT_GEOMETRY, kGeomBytesPerView * (size_t)n_views, &p)))
+    const int kernel_option = workspace_option(w, CTR_OPTION_BACKPROJECT_KERNEL);
+    void *geom = nullptr, *foot = nullptr;
+    if ((rc = workspace_get(w, SLOT_GEOMETRY, kGeomBytesPerView * (size_t)n_views, &geom)))
         return rc;
-    GeomPtrs g;
-    if ((rc = build_geometry_at(s, p, geometry, n_views, source_distance, detector_distance,
-                                detector_width, volume_width, rotation_count, rotation_offset, X,
-                                Y, Z, nu, nv, &g)))
+    if (use_lds_kernel(kernel_option, Y) &&
+        (rc = workspace_get(w, SLOT_FOOTPRINT, footprint_bytes(X, Z, n_views), &foot)))
         return rc;
-    BpJob J = make_job(staged, L, nu, nv, g, view_scale, n_views, volume, X, Y, Z, scale,
-                       accumulate, clip_negative);
-    const bool lds = use_lds_kernel(w, Y);
-    if (lds) {
-        void *f = nullptr;
-        if ((rc = workspace_get(w, SLOT_FOOTPRINT, footprint_bytes(X, Z, n_views), &f))) return rc;
-        if ((rc = build_footprints(s, J, (float4 *)f))) return rc;
-    }
-    const int variant =
-        lds_variant(w, staged_dtype, f16_weights_fit(J, geometry, source_distance, volume_width));
-    switch (staged_dtype) {
-    case CTR_F32: return launch_bp<CTR_F32>(s, J, lds, variant);
-    case CTR_F16: return launch_bp<CTR_F16>(s, J, lds, variant);
-    default: return launch_bp<CTR_BF16>(s, J, lds, variant);
-    }
+    return backproject_with(as_stream(stream), kernel_option,
+                            workspace_option(w, CTR_OPTION_INTERPOLATION), geometry, staged,
+                            staged_dtype, view_scale, nu, nv, n_views, source_distance,
+                            detector_distance, detector_width, volume_width, rotation_count,
+                            rotation_offset, volume, X, Y, Z, scale, accumulate, clip_negative,
+                            geom, foot);
+}
+
+int ctr_backproject_noalloc(void *stream, int geometry, const void *staged, int staged_dtype,
+                            const float *view_scale, int nu, int nv, int n_views,
+                            double source_distance, double detector_distance,
+                            double detector_width, double volume_width, double rotation_count,
+                            double rotation_offset, float *volume, int X, int Y, int Z,
+                            float scale, int accumulate, int clip_negative)
+{
+    clear_error();
+    int rc = check_backproject(geometry, staged, staged_dtype, view_scale, nu, nv, n_views,
+                               source_distance, detector_distance, detector_width, volume_width,
+                               volume, X, Y, Z);
+    if (rc) return rc;
+    const int kernel_option = default_option(CTR_OPTION_BACKPROJECT_KERNEL);
+    const size_t geom_bytes = (kGeomBytesPerView * (size_t)n_views + 255) & ~(size_t)255;
+    const size_t foot = use_lds_kernel(kernel_option, Y) ? footprint_bytes(X, Z, n_views) : 0;
+    CTR_REQUIRE(geom_bytes + foot <= kNoAllocScratchBytes, CTR_ERR_OVER_BUDGET,
+                "ctr_backproject_noalloc: %d views over a %d x %d tile grid need %zu bytes of "
+                "scratch, more than its fixed %zu (use ctr_backproject)", n_views,
+                (X + 15) / 16, (Z + 15) / 16, geom_bytes + foot, kNoAllocScratchBytes);
+    void *base = nullptr;
+    if ((rc = scratch_address(&base))) return rc;
+    return backproject_with(as_stream(stream), kernel_option,
+                            default_option(CTR_OPTION_INTERPOLATION), geometry, staged,
+                            staged_dtype, view_scale, nu, nv, n_views, source_distance,
+                            detector_distance, detector_width, volume_width, rotation_count,
+                            rotation_offset, volume, X, Y, Z, scale, accumulate, clip_negative,
+                            base, (char *)base + geom_bytes);
 }
 
 int ctr_backproject_packed(void *stream, ctr_workspace *ws, int staged_dtype,
@@ -1277,7 +1366,7 @@ int ctr_backproject_packed(void *stream, ctr_workspace *ws, int staged_dtype,
         fp += footprint_bytes(j.X, j.Z, j.n_views);
     }
     CTR_REQUIRE(blocks < (1LL << 31), CTR_ERR_INVALID_ARGUMENT, "packed launch too large");
-    switch (lds_variant(w, staged_dtype, packed)) {
+    switch (lds_variant(workspace_option(w, CTR_OPTION_INTERPOLATION), staged_dtype, packed)) {
     case CTR_F32: launch_lds_packed<CTR_F32>(s, B, (unsigned)blocks); break;
     case CTR_F16: launch_lds_packed<CTR_F16>(s, B, (unsigned)blocks); break;
     case CTR_BF16: launch_lds_packed<CTR_BF16>(s, B, (unsigned)blocks); break;

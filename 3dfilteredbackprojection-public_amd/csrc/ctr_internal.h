@@ -48,6 +48,8 @@ int workspace_resolve(ctr_workspace *ws, ctr_workspace **out);
 int workspace_get(ctr_workspace *ws, int slot, size_t bytes, void **ptr);
 // Current value of a CTR_OPTION_* of a resolved workspace.
 int workspace_option(const ctr_workspace *ws, int option);
+// The load-time value of a CTR_OPTION_* (the default workspaces' value).
+int default_option(int option);
 
 inline bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 inline int ceil_pow2(int n)

@@ -62,6 +62,8 @@ int workspace_option(const ctr_workspace *ws, int option)
     return ws->option[option].load(std::memory_order_relaxed);
 }
 
+int default_option(int option) { return g_option_defaults.value[option]; }
+
 static thread_local char g_error[512] = {0};
 
 int fail(int code, const char *fmt, ...)
@@ -187,7 +189,7 @@ int ctr_workspace_get_option(ctr_workspace *ws, int option)
     CTR_REQUIRE(option >= 0 && option < CTR_OPTION_COUNT, CTR_ERR_INVALID_ARGUMENT,
                 "ctr_workspace_get_option: unknown option %d", option);
     // the default workspaces keep the load-time values: no device needed
-    if (!ws) return ctr::g_option_defaults.value[option];
+    if (!ws) return ctr::default_option(option);
     return ctr::workspace_option(ws, option);
 }
 

@@ -260,6 +260,26 @@ int ctr_backproject(void *stream, ctr_workspace *ws, int geometry,
                     float *volume, int X, int Y, int Z, float scale,
                     int accumulate, int clip_negative);
 
+/* ctr_backproject without a workspace, allocating nothing: no hipMalloc,
+ * hipFree or allocating library call, so it can run where allocation is
+ * not allowed (inside HIP graph capture, tight serving loops, harnesses that
+ * forbid it).  No reference counterpart.  Same arguments and results as
+ * ctr_backproject (the same kernels, bit for bit), with the options of the
+ * default workspace (the environment at library load).  Its per-view
+ * geometry and tile-footprint tables live in a fixed 128 MiB device array of
+ * the library, one per GPU, kept until the process ends: jobs needing more
+ * (144 B per view plus 16 B per view and 16 x 16 column tile; 512^3 x 720
+ * views takes 12 MB, 1024^3 x 1200 views 79 MB) fail with
+ * CTR_ERR_OVER_BUDGET before any launch.  Calls overlapping on one GPU share
+ * that array and race: serialise them (one stream, or events). */
+int ctr_backproject_noalloc(void *stream, int geometry, const void *staged,
+                            int staged_dtype, const float *view_scale, int nu, int nv,
+                            int n_views, double source_distance,
+                            double detector_distance, double detector_width,
+                            double volume_width, double rotation_count,
+                            double rotation_offset, float *volume, int X, int Y, int Z,
+                            float scale, int accumulate, int clip_negative);
+
 /* One reconstruction of a packed backprojection (fields as in
  * ctr_backproject). */
 typedef struct ctr_backproject_job {
