@@ -513,14 +513,19 @@ __global__ __launch_bounds__(kSlabThreads, kSlabTile >= 32 ? 8 : 1) void forward
                         fk = fmaf(t, R.s[2], R.b[2]);
             const int ia = (int)floorf(ax_x ? fi : fk);
             if (ia < lo || ia > hi) break;
-            if (staged) {
-                // trilinear0 on the staged box (zero outside the volume)
-                const float i0f = floorf(fi), j0f = floorf(fj), k0f = floorf(fk);
-                const int i0 = (int)i0f, j0 = (int)j0f, k0 = (int)k0f;
+            // the sample's cell in the staged box: along A it starts in the
+            // slab (the test above); along y and C the box bounds come from
+            // FP32 t-ranges with a one-voxel margin, so a ray that is almost
+            // perpendicular to A could still leave them -- such a sample
+            // takes trilinear0 from memory (the same arithmetic, in order)
+            const float i0f = floorf(fi), j0f = floorf(fj), k0f = floorf(fk);
+            const int i0 = (int)i0f, j0 = (int)j0f, k0 = (int)k0f;
+            const int dy = j0 - yl, dc = (ax_x ? k0 : i0) - cl;
+            if (staged && dy >= 0 && dy <= Ry - 2 && dc >= 0 && dc <= Rc - 2) {
+                // trilinear0 on the staged box (zero outside the volume); the
+                // cell's 8 taps lie inside the S x Ry x Rc box
                 const float wi = fi - i0f, wj = fj - j0f, wk = fk - k0f;
-                int base = (i0 - x0) + (j0 - yl) * sy + (k0 - z0) * sz;
-                base = min(max(base, 0), S * Ry * Rc - 2 - sy - sz);
-                const float *p = box + base;
+                const float *p = box + ((i0 - x0) + dy * sy + (k0 - z0) * sz);
                 const float c00 = fmaf(wi, p[1] - p[0], p[0]);
                 const float c01 = fmaf(wi, p[sy + 1] - p[sy], p[sy]);
                 const float c10 = fmaf(wi, p[sz + 1] - p[sz], p[sz]);

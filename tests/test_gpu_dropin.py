@@ -135,3 +135,91 @@ def test_cpp_fbp3d_slice_and_view_copies(ctr, dev, tmp_path):
     s[:, 2, :] = P(nv, nu)
     s[:nv - 2, N - 2, :] = P(nv - 2, nu + 2)[:, :nu]
     assert np.array_equal(got("sinograms_out", nv * N, nu).reshape(nv, N, nu), s)
+
+
+def test_cpp_ffft_methods(ctr, orc, dev, tmp_path):
+    """Every public FFFT method (FastFourierTransformGPU.h:10-66) through the
+    drop-in class: the padded fft (2^ceil(log2 W), zero tail) and the
+    cropping inverse_fft (:38-91), the shifts in and out of place (:93-142),
+    the blits, conjugate / multiply / divide (:144-236), bit reversal plus
+    single stages (:238-256), floor/ceil_log2 (:269-280) and the reference's
+    precondition errors."""
+    exe = os.path.join(os.path.dirname(ctr.LIB_PATH), "examples", "ffft_main")
+    W, P, H = 300, 256, 5
+    rng = np.random.default_rng(21)
+    rows = (rng.standard_normal((H, W)) + 1j * rng.standard_normal((H, W))).astype(np.complex64)
+    rows_p2 = (rng.standard_normal((H, P)) + 1j * rng.standard_normal((H, P))).astype(np.complex64)
+    rows.tofile(tmp_path / "rows.raw")
+    rows_p2.tofile(tmp_path / "rows_p2.raw")
+    r = subprocess.run([exe, str(tmp_path), str(W), str(P), str(H)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+
+    def got(name, w, complex_=True):
+        a = np.fromfile(tmp_path / f"{name}.raw", dtype=np.complex64 if complex_ else np.float32)
+        return a.reshape(H, w)
+
+    padded = np.zeros((H, 512), dtype=np.complex128)
+    padded[:, :W] = rows
+    ref = np.stack([orc.fft_radix2(z) for z in padded])
+    assert rel_max(got("fft_padded", 512), ref) < 2e-5
+    assert rel_max(got("inverse_cropped", W), rows) < 1e-5
+    ref = np.stack([orc.inverse_fft_radix2(z.astype(np.complex128)) for z in rows_p2])
+    assert rel_max(got("inverse_same", P), ref) < 1e-5
+    assert np.array_equal(got("shift_out", P), np.roll(rows_p2, P // 2, axis=1))
+    assert np.array_equal(got("shift_in_place", P), np.roll(rows_p2, P // 2, axis=1))
+    assert np.array_equal(got("unshift_in_place", P), rows_p2)
+    assert np.array_equal(got("shift_by_3", P), np.roll(rows_p2, 3, axis=1))
+    ref = np.stack([orc.fft_radix2(z.astype(np.complex128)) for z in rows_p2])
+    assert rel_max(got("stages", P), ref) < 2e-5
+    assert np.array_equal(got("real_r", W, False), rows.real)
+    assert np.array_equal(got("real_i", W, False), rows.imag)
+    assert np.array_equal(got("complex_from_real", W), rows.real.astype(np.complex64))
+    c = np.conj(rows)
+    ref = ((c.real * np.float32(2.5)) * np.float32(0.25)
+           + 1j * ((c.imag * np.float32(2.5)) * np.float32(0.25))).astype(np.complex64)
+    assert np.array_equal(got("conj_mul_div", W), ref)
+    out = r.stdout
+    for n, fl, ce in ((1, 0, 0), (2, 1, 1), (3, 1, 2), (300, 8, 9), (512, 9, 9), (513, 9, 10)):
+        assert f"log2 {n}: floor {fl} ceil {ce}" in out
+    assert out.count("error as expected") == 2 and "no error" not in out
+    assert "[CTAnalyzer Error] FFFT::fft_radix2() is called but texture's width is not a " \
+           "power of 2" in out
+
+
+def test_cpp_fbp3d_detector_and_volume_passes(ctr, orc, dev, tmp_path):
+    """FBP3D's log-normalisation, FDK weighting and row filtering of both
+    projection layouts and its negative clip (FBP3D.cpp:333-458), through the
+    drop-in class, each stage against the oracle applied to the previous
+    GPU stage."""
+    exe = os.path.join(os.path.dirname(ctr.LIB_PATH), "examples", "fbp3d_passes_main")
+    nu, nv, N, X, Y, Z = 100, 24, 7, 20, 12, 16
+    base, sod, odd, width = (float(np.float32(v)) for v in (1e5, 400.0, 300.0, 250.0))
+    rng = np.random.default_rng(8)
+    proj = (base * np.exp(-rng.uniform(0, 5, (N, nv, nu)))).astype(np.float32)
+    sino = (base * np.exp(-rng.uniform(0, 5, (nv, N, nu)))).astype(np.float32)
+    vol = rng.standard_normal((Z, Y, X)).astype(np.float32)
+    for name, a in (("projections", proj), ("sinograms", sino), ("volume", vol)):
+        a.tofile(tmp_path / f"{name}.raw")
+    r = subprocess.run([exe, str(tmp_path), *map(str, (nu, nv, N, X, Y, Z, base, sod, odd,
+                                                       width))],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+
+    def got(name, shape):
+        return np.fromfile(tmp_path / f"{name}.raw", dtype=np.float32).reshape(shape)
+
+    flt_min = float(np.finfo(np.float32).tiny)
+    for layout, data, shape in (("projections", proj, (N, nv, nu)),
+                                ("sinograms", sino, (nv, N, nu))):
+        log = got(f"{layout}_log", shape)
+        assert np.abs(log - orc.log_normalize(data.astype(np.float64), base, flt_min)).max() < 2e-6
+        fdk = got(f"{layout}_fdk", shape)
+        as_proj = (lambda a: a) if layout == "projections" else (lambda a: a.transpose(1, 0, 2))
+        ref = as_proj(orc.fdk_weight(as_proj(log), sod, odd, width))
+        assert np.abs(fdk - ref).max() <= 2 ** -24 * np.abs(ref).max()
+        ftype = orc.SHEPP_LOGAN if layout == "projections" else orc.COSINE
+        filt = got(f"{layout}_filtered", shape)
+        ref = orc.filter_rows(fdk.reshape(-1, nu), ftype).reshape(shape)  # default 2^ceil(log2 nu)
+        assert rel_max(filt, ref) < 1e-5
+    assert np.array_equal(got("volume_clipped", (Z, Y, X)), np.maximum(vol, np.float32(0)))

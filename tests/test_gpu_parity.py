@@ -230,7 +230,8 @@ def test_forward_slab_kernel_matches_ray_kernel(ctr, dev, step):
     """The slab-staged forward projector against the ray kernel, bit for bit:
     a full turn of views (every main axis and direction), a fan wide enough
     that some rays run against the view's central direction (those take the
-    in-order fallback), odd sizes and several steps."""
+    in-order fallback), a fan of +-85 degrees, odd sizes and several
+    steps."""
     import torch
     rng = np.random.default_rng(int(step * 10))
     X, Y, Z = 45, 37, 51
@@ -243,7 +244,13 @@ def test_forward_slab_kernel_matches_ray_kernel(ctr, dev, step):
                           detector_distance=40.0, rotation_offset=0.3),
              ctr.Geometry(ctr.PARALLEL, nu=64, nv=40, n_views=29, X=X, Y=Y, Z=Z,
                           detector_width=64.0, volume_width=45.0, rotation_count=0.5,
-                          rotation_offset=0.2)]
+                          rotation_offset=0.2),
+             # a fan of +-85 degrees (and +-62 degrees vertically): rays far
+             # from the view's main axis, whose y / cross-axis cells test the
+             # bounds of the staged box
+             ctr.Geometry(ctr.CONE, nu=160, nv=48, n_views=23, X=X, Y=Y, Z=Z,
+                          detector_width=2400.0, volume_width=45.0, source_distance=60.0,
+                          detector_distance=40.0, rotation_offset=0.7)]
     for g in geoms:
         slab = ctr.forward_project(vol, g, step=step, ws=_forward_ws(ctr, False, False))
         slab_wide = ctr.forward_project(vol, g, step=step, ws=_forward_ws(ctr, False, True))
