@@ -755,21 +755,22 @@ static int prepare_ellipsoid(const double *P, int e, EllipsoidRec *Q)
 // sum_e density_e * chord_e along each detector ray, the rays of
 // ctr_forward_project (source SOD (cos t, 0, sin t), flat detector at the
 // detector distance beyond the axis; parallel rays along -(cos t, 0, sin t)
-// through u (-sin t, 0, cos t) + v y).  One thread per ray.  Each ellipsoid
-// is first culled in FP32 against its bounding sphere (distance of the
-// centre from the ray, with a margin far above FP32 rounding); the chord of
-// a candidate is FP64: the ray is moved to its point nearest the centre,
-// taken to the unit-sphere body frame, and the chord is
-// 2 sqrt(B^2 - A C) / A of |r + s q|^2 = 1.  The records travel in the kernel
-// argument (no device table, no host wait): at most kEllGroup per launch.
+// through u (-sin t, 0, cos t) + v y).  One workgroup per 16 x 16 pixel tile
+// of one view: one thread computes the view's sine and cosine and the
+// tile's ray cone, one thread per ellipsoid culls its bounding sphere
+// against that cone (FP64, conservative), and every ray then takes only the
+// candidates of its tile.  The chord is FP64: the ray is moved to its point
+// nearest the centre, taken to the unit-sphere body frame, and the chord is
+// 2 sqrt(B^2 - A C) / A of |r + s q|^2 = 1.  The records travel in the
+// kernel argument (no device table, no host wait): kEllGroup per launch.
 // ---------------------------------------------------------------------------
-constexpr int kEllGroup = 28;
+constexpr int kEllGroup = 32;  // one bit each in the tile's candidate mask
+constexpr int kEllTile = 16;
 struct EllipsoidGroup {
     double c[kEllGroup][3];
     double m[kEllGroup][9];
     double density[kEllGroup];
-    float cf[kEllGroup][3];
-    float reach2[kEllGroup];  // (bounding radius + margin)^2, FP32 cull
+    double radius[kEllGroup];  // bounding sphere: the largest semi-axis
     int count;
 };
 struct RayGeometry {
@@ -779,34 +780,113 @@ struct RayGeometry {
 };
 static_assert(sizeof(EllipsoidGroup) + sizeof(RayGeometry) < 4000, "kernel argument size");
 
+__device__ inline double dot3(const double *a, const double *b)
+{
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+// Unit direction from the source to detector point (u, v) of a cone view.
+__device__ inline void cone_ray(const RayGeometry &G, double st, double ct, double u, double v,
+                                double *d)
+{
+    d[0] = -G.odd * ct - u * st - G.sod * ct;
+    d[1] = v;
+    d[2] = -G.odd * st + u * ct - G.sod * st;
+    const double inv = 1.0 / sqrt(dot3(d, d));
+    d[0] *= inv;
+    d[1] *= inv;
+    d[2] *= inv;
+}
+
 __global__ __launch_bounds__(256) void project_ellipsoids_kernel(EllipsoidGroup E, RayGeometry G)
 {
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long total = (long long)G.nu * G.nv * G.n_views;
-    if (idx >= total) return;
-    const int iu = (int)(idx % G.nu);
-    const long long r = idx / G.nu;
-    int iv, n;
-    if (G.layout == CTR_LAYOUT_PROJECTIONS) {
-        iv = (int)(r % G.nv);
-        n = (int)(r / G.nv);
-    } else {
-        n = (int)(r % G.n_views);
-        iv = (int)(r / G.n_views);
+    // view trig; the tile's ray cone (cone beam: axis, cos and sin of its
+    // half-angle) or pixel-centre rectangle (parallel beam)
+    __shared__ double tile[8];
+    __shared__ unsigned cand;
+    const int tu = (G.nu + kEllTile - 1) / kEllTile, tv = (G.nv + kEllTile - 1) / kEllTile;
+    const int n = (int)(blockIdx.x / (unsigned)(tu * tv));
+    const int t = (int)(blockIdx.x % (unsigned)(tu * tv));
+    const int u0 = (t % tu) * kEllTile, v0 = (t / tu) * kEllTile;
+    const int u1 = min(u0 + kEllTile, G.nu) - 1, v1 = min(v0 + kEllTile, G.nv) - 1;
+    const bool cone = G.geometry == CTR_GEOMETRY_CONE;
+    if (threadIdx.x == 0) {
+        // FBP3D.cpp:223-227,283-284: theta_n = 2 pi rot n / N + offset
+        const double th = 2.0 * M_PI * G.rot * (double)n / (double)G.n_views + G.offset;
+        double st, ct;
+        sincos(th, &st, &ct);
+        tile[0] = st;
+        tile[1] = ct;
+        const double ua = (u0 + 0.5 - 0.5 * G.nu) * G.delta, ub = (u1 + 0.5 - 0.5 * G.nu) * G.delta;
+        const double va = (v0 + 0.5 - 0.5 * G.nv) * G.delta, vb = (v1 + 0.5 - 0.5 * G.nv) * G.delta;
+        if (cone) {
+            double axis[3], d[3];
+            cone_ray(G, st, ct, 0.5 * (ua + ub), 0.5 * (va + vb), axis);
+            double cmin = 1.0;
+            for (int k = 0; k < 4; k++) {
+                cone_ray(G, st, ct, k & 1 ? ub : ua, k & 2 ? vb : va, d);
+                cmin = fmin(cmin, dot3(axis, d));
+            }
+            tile[2] = axis[0];
+            tile[3] = axis[1];
+            tile[4] = axis[2];
+            tile[5] = cmin;                                   // cos(beta)
+            tile[6] = sqrt(fmax(0.0, 1.0 - cmin * cmin));     // sin(beta)
+        } else {
+            tile[2] = ua;
+            tile[3] = ub;
+            tile[4] = va;
+            tile[5] = vb;
+        }
+        cand = 0u;
     }
-    // FBP3D.cpp:223-227,283-284: theta_n = 2 pi rot n / N + offset
-    const double th = 2.0 * M_PI * G.rot * (double)n / (double)G.n_views + G.offset;
-    double st, ct;
-    sincos(th, &st, &ct);
+    __syncthreads();
+    const double st = tile[0], ct = tile[1];
+    if ((int)threadIdx.x < E.count) {
+        const int e = threadIdx.x;
+        const double R = E.radius[e] * (1.0 + 1e-9) + 1e-9;
+        bool hit;
+        if (cone) {
+            // a line of the tile's ray cone can meet the sphere only if the
+            // angle between the cone axis and the centre direction is at most
+            // alpha + beta (alpha the sphere's angular radius)
+            const double w[3] = {E.c[e][0] - G.sod * ct, E.c[e][1], E.c[e][2] - G.sod * st};
+            const double L = sqrt(dot3(w, w));
+            if (L <= R) {
+                hit = true;
+            } else {
+                const double axis[3] = {tile[2], tile[3], tile[4]};
+                const double sa = R / L, ca = sqrt(1.0 - sa * sa);
+                const double cb = tile[5], sb = tile[6];
+                const double cg = dot3(axis, w) / L;
+                // cos(alpha + beta), or -1 once alpha + beta reaches pi
+                const double cab = (ca * cb - sa * sb < -1.0 || (ca < 0.0 && cb < 0.0))
+                                       ? -1.0
+                                       : ca * cb - sa * sb;
+                // a line integral: the cone about -axis counts too
+                hit = fabs(cg) >= cab - 1e-12;
+            }
+        } else {
+            // nearest point of the tile's pixel-centre rectangle to the
+            // centre's detector coordinates
+            const double uc = -E.c[e][0] * st + E.c[e][2] * ct, vc = E.c[e][1];
+            const double du = fmax(fmax(tile[2] - uc, uc - tile[3]), 0.0);
+            const double dv = fmax(fmax(tile[4] - vc, vc - tile[5]), 0.0);
+            hit = du * du + dv * dv <= R * R;
+        }
+        if (hit) atomicOr(&cand, 1u << e);
+    }
+    __syncthreads();
+    unsigned mask = cand;
+    const int iu = u0 + (int)(threadIdx.x % kEllTile), iv = v0 + (int)(threadIdx.x / kEllTile);
+    if (iu >= G.nu || iv >= G.nv) return;
     const double u = (iu + 0.5 - 0.5 * G.nu) * G.delta, v = (iv + 0.5 - 0.5 * G.nv) * G.delta;
     double o[3], d[3];
-    if (G.geometry == CTR_GEOMETRY_CONE) {
+    if (cone) {
         o[0] = G.sod * ct;
         o[1] = 0.0;
         o[2] = G.sod * st;
-        d[0] = -G.odd * ct - u * st - o[0];
-        d[1] = v;
-        d[2] = -G.odd * st + u * ct - o[2];
+        cone_ray(G, st, ct, u, v, d);
     } else {
         o[0] = -u * st;
         o[1] = v;
@@ -815,20 +895,11 @@ __global__ __launch_bounds__(256) void project_ellipsoids_kernel(EllipsoidGroup 
         d[1] = 0.0;
         d[2] = -st;
     }
-    const double inv = 1.0 / sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    d[0] *= inv;
-    d[1] *= inv;
-    d[2] *= inv;
-    const float of[3] = {(float)o[0], (float)o[1], (float)o[2]};
-    const float df[3] = {(float)d[0], (float)d[1], (float)d[2]};
     double acc = 0.0;
-    for (int e = 0; e < E.count; e++) {
-        // cull: squared distance of the centre from the ray, FP32
-        const float wx = E.cf[e][0] - of[0], wy = E.cf[e][1] - of[1], wz = E.cf[e][2] - of[2];
-        const float t = wx * df[0] + wy * df[1] + wz * df[2];
-        const float h2 = wx * wx + wy * wy + wz * wz - t * t;
-        if (h2 > E.reach2[e]) continue;
-        // FP64 chord: p = point of the ray nearest the centre, minus the centre
+    while (mask) {
+        const int e = __ffs(mask) - 1;
+        mask &= mask - 1;
+        // p = point of the ray nearest the centre, minus the centre
         const double w0 = E.c[e][0] - o[0], w1 = E.c[e][1] - o[1], w2 = E.c[e][2] - o[2];
         const double t0 = w0 * d[0] + w1 * d[1] + w2 * d[2];
         const double p0 = t0 * d[0] - w0, p1 = t0 * d[1] - w1, p2 = t0 * d[2] - w2;
@@ -844,6 +915,9 @@ __global__ __launch_bounds__(256) void project_ellipsoids_kernel(EllipsoidGroup 
         const double disc = B * B - A * C;
         if (disc > 0.0) acc += E.density[e] * 2.0 * sqrt(disc) / A;
     }
+    const size_t idx = G.layout == CTR_LAYOUT_PROJECTIONS
+                           ? ((size_t)n * G.nv + iv) * G.nu + iu
+                           : ((size_t)iv * G.n_views + n) * G.nu + iu;
     float *dst = G.out + idx;
     *dst = G.accumulate ? *dst + (float)acc : (float)acc;
 }
@@ -1049,8 +1123,9 @@ int ctr_project_ellipsoids(void *stream, int geometry, const double *ellipsoids,
                 CTR_ERR_INVALID_ARGUMENT, "ctr_project_ellipsoids: bad detector");
     CTR_REQUIRE(geometry == CTR_GEOMETRY_PARALLEL || (source_distance > 0 && detector_distance >= 0),
                 CTR_ERR_INVALID_ARGUMENT, "ctr_project_ellipsoids: bad cone geometry");
-    const long long total = (long long)nu * nv * n_views;
-    CTR_REQUIRE(total < (1LL << 40), CTR_ERR_INVALID_ARGUMENT, "too many rays");
+    const long long blocks = (long long)((nu + kEllTile - 1) / kEllTile) *
+                             ((nv + kEllTile - 1) / kEllTile) * n_views;
+    CTR_REQUIRE(blocks < (1LL << 31), CTR_ERR_INVALID_ARGUMENT, "too many rays");
     RayGeometry G;
     G.geometry = geometry;
     G.layout = layout;
@@ -1072,22 +1147,13 @@ int ctr_project_ellipsoids(void *stream, int geometry, const double *ellipsoids,
             EllipsoidRec R;
             int rc = prepare_ellipsoid(P, e0 + q, &R);
             if (rc) return rc;
-            for (int a = 0; a < 3; a++) {
-                E.c[q][a] = R.c[a];
-                E.cf[q][a] = (float)R.c[a];
-            }
+            for (int a = 0; a < 3; a++) E.c[q][a] = R.c[a];
             for (int a = 0; a < 9; a++) E.m[q][a] = R.m[a];
             E.density[q] = R.density;
-            // squared bounding radius plus a margin: the FP32 cull's h2 =
-            // |w|^2 - t^2 carries an absolute rounding of a few 2^-24 |w|^2,
-            // |w| <= `far`; 1e-6 far^2 is over 16 times that
-            const double far = fabs(R.c[0]) + fabs(R.c[1]) + fabs(R.c[2]) + source_distance +
-                               detector_distance + detector_width;
-            const double rad = fmax(P[3], fmax(P[4], P[5]));
-            E.reach2[q] = (float)(rad * rad * (1.0 + 1e-4) + 1e-6 * far * far);
+            E.radius[q] = fmax(P[3], fmax(P[4], P[5]));
         }
         G.accumulate = accumulate || e0 > 0;
-        project_ellipsoids_kernel<<<grid1d(total, 256), 256, 0, s>>>(E, G);
+        project_ellipsoids_kernel<<<(unsigned)blocks, 256, 0, s>>>(E, G);
         CTR_CHECK_LAUNCH();
         if (count == 0) break;
     }

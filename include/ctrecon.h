@@ -356,8 +356,9 @@ int ctr_render_ellipsoids(void *stream, ctr_workspace *ws,
 /* Exact line integrals of ellipsoids (records as in ctr_render_ellipsoids)
  * along the rays of ctr_forward_project: p = sum_e density_e * chord_e,
  * written to (accumulate = 0) or added to (accumulate != 0) `out` in the
- * given layout.  The chords are FP64; each ellipsoid is first culled per
- * ray against its bounding sphere in FP32 with a safe margin.  Makes the
+ * given layout.  The chords are FP64; each 16 x 16 pixel tile of a view
+ * first culls the ellipsoids whose bounding sphere no line of the tile can
+ * meet (FP64, conservative).  Makes the
  * seeded input of each serving request without a voxelised phantom.  No
  * reference counterpart (the reference projects voxel volumes,
  * FBP3D.cpp:269-330). */

@@ -230,3 +230,40 @@ def test_render_rotation_convention(orc):
     needle = np.array([[0.0, 0.0, 0.0, 10.0, 1.2, 1.2, 0.0, 0.0, math.pi / 2, 1.0]])
     vol = orc.render_ellipsoids(needle, (32, 32, 32), 32.0)
     assert vol[16, :, 16].sum() > 15 and vol[16, 16, :].sum() < 4
+
+
+# --- the closed-form helpers ----------------------------------------------------
+
+@pytest.mark.parametrize("n", [1, 3, 12, 64, 100])
+def test_direct_dft_matches_numpy(orc, n):
+    """The long-double DFT that pins the true-pi FFT, against numpy's FFT
+    (any n, forward and backward)."""
+    rng = np.random.default_rng(n)
+    z = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    assert np.abs(orc.dft(z, -1) - np.fft.fft(z)).max() < 1e-12 * max(1, n)
+    assert np.abs(orc.dft(z, +1) - n * np.fft.ifft(z)).max() < 1e-12 * max(1, n)
+
+
+def test_analytic_cone_sphere_closed_form(orc):
+    """analytic_cone's ray for detector pixel (iu, iv) of view n, rebuilt
+    from vectors (source SOD e, pixel at -ODD e + u eu + v y): a sphere's
+    projection is 2 sqrt(r^2 - h^2) times its density, h the distance of its
+    centre from that ray."""
+    sod, odd, det_w, nu, nv, N = 300.0, 200.0, 240.0, 24, 18, 5
+    c, r, rho = np.array([12.0, -7.0, 5.0]), 30.0, 0.7
+    ball = np.array([[*c, r, r, r, 0.3, 0.2, 0.1, rho]])
+    p = orc.analytic_cone(ball, sod, odd, det_w, nu, nv, N, 1.0, 0.25)
+    delta = det_w / nu
+    for n, iv, iu in [(0, 9, 12), (2, 3, 5), (4, 15, 20), (3, 0, 0)]:
+        th = 2 * math.pi * n / N + 0.25
+        e = np.array([math.cos(th), 0.0, math.sin(th)])
+        eu = np.array([-math.sin(th), 0.0, math.cos(th)])
+        u, v = (iu + 0.5 - nu / 2) * delta, (iv + 0.5 - nv / 2) * delta
+        S = sod * e
+        D = -odd * e + u * eu + np.array([0.0, v, 0.0])
+        d = (D - S) / np.linalg.norm(D - S)
+        w = c - S
+        h2 = np.dot(w, w) - np.dot(w, d) ** 2
+        ref = rho * 2 * math.sqrt(max(r * r - h2, 0.0))
+        assert p[n, iv, iu] == pytest.approx(ref, abs=1e-9)
+    assert p.max() > 2 * rho * r * 0.9
