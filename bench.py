@@ -19,8 +19,9 @@ profiles/backproject_fdk512_sq_counters.md, times the updates of one launch)
 over the launch's duration (HIP events on its stream), against the issue
 rate of MI355X's published 157.3 TFLOP/s FP32 vector peak: 157.3e12 / 2
 FLOP per FMA / 64 lanes = 1229 G wave-instructions/s.  That published rate
-counts packed FP32 (two FMAs per lane per instruction); without it, which
-this build does not use, the ceiling is half that.  The HBM side is reported
+counts packed FP32 (two FMAs per lane per instruction), which this build
+does not use; beside it the line gives the rate a probe of the kernel's own
+instruction mix reaches on the box (profiles/).  The HBM side is reported
 too: bytes measured per launch (FETCH_SIZE + WRITE_SIZE, scaled by a copy of
 known size, profiles/backproject_fdk512_traffic.md) over the same duration,
 against 8 TB/s, and the tap bytes the LDS serves.
@@ -318,6 +319,10 @@ def main():
 
     v16, achieved16 = valu("fp16", bp_ms)
     v32, achieved32 = valu("fp32", bp_ms32)
+    # the issue rate a probe of the fp16 loop's own instruction mix reaches
+    # (wave64 VALU instructions per ns per SIMD, independent chains)
+    mix_ceiling = profile_value("backproject_fdk512_sq_counters.md",
+                                "fp16_mix_issue_ceiling_per_ns")
     traffic16 = profile_value("backproject_fdk512_traffic.md", "fp16_bytes_per_launch")
     traffic32 = profile_value("backproject_fdk512_traffic.md", "fp32_bytes_per_launch")
     tap_bytes16 = (4 * 2 + 8.0 / g.n_views) * g.updates
@@ -402,8 +407,9 @@ def main():
             "peak": round(VALU_PEAK_GINSTR, 1),
             "unit": "G wave64 VALU instructions/s",
             "frac": round(achieved16 / VALU_PEAK_GINSTR, 4) if achieved16 else None,
-            "frac_of_unpacked_issue": (round(2 * achieved16 / VALU_PEAK_GINSTR, 4)
-                                       if achieved16 else None),
+            "issue_per_ns_per_simd": (round(achieved16 / 1024, 3) if achieved16 else None),
+            "frac_of_mix_ceiling": (round(achieved16 / 1024 / mix_ceiling, 4)
+                                    if achieved16 and mix_ceiling else None),
             "valu_per_update": v16,
             "kernel": "backproject_lds_kernel<fp16, 32> (FP32 interpolation; LDS 48 columns, "
                       "pitch 51, 72 rows)",
