@@ -20,8 +20,6 @@
 // A 256-thread block is 4 wave64s, each covering an 8 x 8 patch of columns,
 // so the 64 lanes of a wave sample a compact detector footprint.
 #include <mutex>
-#include <cstdlib>
-#include <cstring>
 
 #include "ctr_internal.h"
 

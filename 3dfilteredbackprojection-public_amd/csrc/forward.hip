@@ -1,8 +1,6 @@
 // forward.hip -- ray-driven forward projection, fan->parallel rebinning and
 // the elementwise detector / volume passes, for gfx950.
 #include <cmath>
-#include <cstdlib>
-#include <cstring>
 #include <vector>
 
 #include "ctr_internal.h"
@@ -1139,7 +1137,11 @@ int ctr_project_ellipsoids(void *stream, int geometry, const double *ellipsoids,
     G.offset = rotation_offset;
     G.out = out;
     hipStream_t s = as_stream(stream);
-    for (int e0 = 0; e0 < count || (e0 == 0 && !accumulate); e0 += kEllGroup) {
+    // groups of kEllGroup ellipsoids, each launch adding to the previous
+    // ones; with none, one launch still clears `out` unless accumulating
+    const int groups = count > 0 ? (count + kEllGroup - 1) / kEllGroup : (accumulate ? 0 : 1);
+    for (int gi = 0; gi < groups; gi++) {
+        const int e0 = gi * kEllGroup;
         EllipsoidGroup E;
         E.count = count - e0 < kEllGroup ? count - e0 : kEllGroup;
         for (int q = 0; q < E.count; q++) {
@@ -1152,10 +1154,9 @@ int ctr_project_ellipsoids(void *stream, int geometry, const double *ellipsoids,
             E.density[q] = R.density;
             E.radius[q] = fmax(P[3], fmax(P[4], P[5]));
         }
-        G.accumulate = accumulate || e0 > 0;
+        G.accumulate = accumulate || gi > 0;
         project_ellipsoids_kernel<<<(unsigned)blocks, 256, 0, s>>>(E, G);
         CTR_CHECK_LAUNCH();
-        if (count == 0) break;
     }
     return CTR_OK;
 }

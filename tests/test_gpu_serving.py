@@ -15,7 +15,7 @@
 import numpy as np
 import pytest
 
-from conftest import rel_max, rel_rmse
+from conftest import error_metrics, format_metrics, rel_max
 
 pytestmark = pytest.mark.gpu
 
@@ -163,8 +163,7 @@ def test_serving_at_real_sizes(ctr, orc, dev):
     ref = np.maximum(ref, 0.0)  # the served pipeline clips negatives
     (i0, i1), (j0, j1), (k0, k1) = box
     got = large.result[k0:k1, j0:j1, i0:i1].cpu().numpy()
-    err = rel_rmse(got, ref)
-    print(f"[c4] 1024^3 crop vs oracle: rel RMSE {err:.2e}, peak {np.abs(ref).max():.3e}",
-          flush=True)
+    m = error_metrics(got, ref)
+    print(f"[c4] 1024^3 crop vs oracle (FP16 store): {format_metrics(m)}", flush=True)
     assert np.abs(ref).max() > 0
-    assert err < 1e-3
+    assert m["rel_rmse"] < 1e-3
