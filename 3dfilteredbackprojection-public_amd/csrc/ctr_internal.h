@@ -39,7 +39,8 @@ enum Slot {
     SLOT_ELLIPSOIDS = 3, // ellipsoid records for the phantom renderer
     SLOT_FOOTPRINT = 4,  // per (tile column, view) detector footprint bounds
     SLOT_REDUCE = 5,     // min / max reduction keys
-    SLOT_COUNT = 6
+    SLOT_VOLUME_T = 6,   // forward projector: the volume with z fastest
+    SLOT_COUNT = 7
 };
 
 // Resolves NULL to the per-device default workspace.

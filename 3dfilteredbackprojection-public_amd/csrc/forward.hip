@@ -316,38 +316,59 @@ __global__ __launch_bounds__(256) void forward_kernel(ForwardJob J)
 }
 
 // ---------------------------------------------------------------------------
-// Slab-staged forward projection.  The block's 256 rays (one 16 x 16 detector
-// tile of one view) march together through slabs of kSlabS slices along the
-// view's main horizontal axis A (x when |cos theta| >= |sin theta|, else z),
-// consecutive slabs sharing one slice.  For each slab the block stages the
-// box of voxels its rays' samples can touch there -- zero outside the volume
-// -- in LDS, and every ray takes the samples whose cell starts in the slab
-// from LDS.  Each ray still sums its samples in order with trilinear0's
-// arithmetic on the same taps, so the result equals forward_kernel's bit for
-// bit.  A slab whose box exceeds the LDS region reads the volume directly, and
-// a ray that does not run along +A (or -A) like the view's central ray, or a
-// sample outside every slab, falls back to trilinear0 after the slabs, in
-// order.
+// Slab-staged forward projection.  The block's 256 rays (one kSlabTU x
+// kSlabTV detector tile of one view) march together through slabs of kSlabS
+// slices along the view's main horizontal axis A (x when |cos theta| >=
+// |sin theta|, else z), consecutive slabs sharing one slice.  For each slab
+// the block stages the box of voxels its rays' samples can touch there --
+// zero outside the volume -- in LDS, and every ray takes the samples whose
+// cell starts in the slab from LDS.  Each ray still sums its samples in order
+// with trilinear0's arithmetic on the same taps, so the result equals
+// forward_kernel's bit for bit.  A slab whose box exceeds the LDS region
+// reads the volume directly, and a ray that does not run along +A (or -A)
+// like the view's central ray, or a sample outside every slab, falls back to
+// trilinear0 after the slabs, in order.
+//
+// The box is staged as [A][y][C], the cross axis C (z when A = x, x when
+// A = z) fastest, and read from memory row by row along C: from the volume
+// itself when C = x, from a copy with z fastest (volT, (x, y, z) at
+// (x Y + y) Z + z) when C = z.  Each box row is then one contiguous run of
+// Rc voxels in memory, and the detector tile's u side -- which spans C --
+// sets its length.
 // ---------------------------------------------------------------------------
-// Slab depth and box capacity.  Measured (ms, 512^3 / 256^3 inputs, all
-// bit-identical): 4 slices / 12 KB 837 / 51.4, 5 / 15 KB 696 / 42.4, 6 /
-// 18 KB 654 / 40.5, 8 / 20 KB 676 / 41.5, 8 / 24 KB 718 / 43.9, 8 / 36 KB
-// 919 / 54.5, 12 / 36 KB 831 / 48.8, 16 / 48 KB 1158 / 68.6; the ray kernel
-// 1163 / 53.1.  Up to 20 KB eight workgroups fit a CU.
+// Slab depth, box capacity (floats) and the detector tile (u x v rays).
+// Measured (ms; 512^3 x 720 views of 768^2, 256^3 x 360 of 384^2, 1440^3 x
+// 96 of 1536^2; all bit-identical): 16 x 16 rays, 6 slices, 18 KB 583 / 35.2
+// / 1242; 32 x 8 623 / 37.4 / 1351; 64 x 4 845 / 46.0 / 1871; 32 x 8 with 8
+// slices in 24 KB 641 / 37.4 / 1420.  The earlier fill (box x fastest, read
+// row by row along x for every view, element indices by two integer
+// divisions) measured 667 / 41.2 / 1393 at 16 x 16.  Before it: 4 slices /
+// 12 KB 837 ms at 512^3, 5 / 15 KB 696, 8 / 20 KB 676, 12 / 36 KB 831,
+// 16 / 48 KB 1158; 32 x 32 rays with 36-48 KB boxes 756-803.
 #ifndef CTR_FWD_SLAB_S
 #define CTR_FWD_SLAB_S 6
 #endif
 #ifndef CTR_FWD_SLAB_CAP
 #define CTR_FWD_SLAB_CAP 4608
 #endif
-// 1: each XCD takes one contiguous eighth of every view's tiles (neighbouring
-// boxes on one L2).  Measured 700 / 43.2 ms against 655 / 40.4 ms for the
-// plain order (512^3 / 256^3 inputs), so it stays off.
-#ifndef CTR_FWD_SLAB_XCD
-#define CTR_FWD_SLAB_XCD 0
+#ifndef CTR_FWD_SLAB_TU
+#define CTR_FWD_SLAB_TU 16
 #endif
 constexpr int kSlabS = CTR_FWD_SLAB_S;      // slices per slab
 constexpr int kSlabCap = CTR_FWD_SLAB_CAP;  // floats of the staged box
+constexpr int kSlabThreads = 256, kSlabWaves = kSlabThreads / 64;
+constexpr int kSlabTU = CTR_FWD_SLAB_TU, kSlabTV = kSlabThreads / kSlabTU;
+static_assert(kSlabTU * kSlabTV == kSlabThreads, "slab tile");
+
+// trilinear0 out of line: the slab kernel's samples outside the staged box
+// are rare, and inlined its eight bounds-checked taps would put a chain of
+// exec-mask branches into every iteration of the sample loop.
+__device__ __attribute__((noinline)) float trilinear0_call(const float *__restrict__ vol, int X,
+                                                            int Y, int Z, float fi, float fj,
+                                                            float fk)
+{
+    return trilinear0(vol, X, Y, Z, fi, fj, fk);
+}
 
 __device__ inline int wave_min(int v)
 {
@@ -360,44 +381,104 @@ __device__ inline int wave_max(int v)
     return v;
 }
 
-// Detector tile of a slab block: kSlabTile x kSlabTile rays, one per thread.
-// 32 x 32 rays share each staged voxel among more rays but measured slower
-// (ms, 512^3 / 256^3 inputs): 36 KB boxes of 6 slices 803 / 49.6, 48 KB 756 /
-// 48.1, 36 KB of 4 slices 1293 / 81.2, against 640 / 39.5 for 16 x 16.
-#ifndef CTR_FWD_SLAB_TILE
-#define CTR_FWD_SLAB_TILE 16
-#endif
-constexpr int kSlabTile = CTR_FWD_SLAB_TILE;
-constexpr int kSlabThreads = kSlabTile * kSlabTile, kSlabWaves = kSlabThreads / 64;
+// (x, y, z) -> (x Y + y) Z + z: the volume with z fastest.  Block (b, g)
+// transposes 32 x 32 (z, x) tile b of the y planes g, g + gridDim.y, ...
+// through LDS.
+__global__ __launch_bounds__(256) void transpose_xz_kernel(const float *__restrict__ vol,
+                                                           float *__restrict__ volT, int X, int Y,
+                                                           int Z)
+{
+    __shared__ float t[32][33];
+    const int tiles_x = (X + 31) / 32;
+    const int x0 = (int)(blockIdx.x % tiles_x) * 32, z0 = (int)(blockIdx.x / tiles_x) * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int y = blockIdx.y; y < Y; y += gridDim.y) {
+        for (int r = ty; r < 32; r += 8) {
+            const int x = x0 + tx, z = z0 + r;
+            if (x < X && z < Z) t[r][tx] = vol[((size_t)z * Y + y) * X + x];
+        }
+        __syncthreads();
+        for (int r = ty; r < 32; r += 8) {
+            const int z = z0 + tx, x = x0 + r;
+            if (x < X && z < Z) volT[((size_t)x * Y + y) * Z + z] = t[tx][r];
+        }
+        __syncthreads();
+    }
+}
+
+// The samples k, k + 1, ... of ray R whose cell starts in slab [lo, hi]
+// along A, trilinear from the staged box ([A][y][C], C = z when AXX, else x,
+// unit stride along C: the taps pair into two-dword LDS reads).  A sample
+// whose cell leaves the box along y or C takes trilinear0 from memory.
+template <bool AXX>
+__device__ inline void slab_samples(const ForwardJob &J, const IndexRay &R, int &k, float &acc,
+                                    const float *box, bool staged, int lo, int hi, int yl, int cl,
+                                    int Ry, int Rc)
+{
+    const unsigned sb_a = (unsigned)(Ry * Rc), sj = (unsigned)Rc;
+    const unsigned ry2 = staged ? (unsigned)(Ry - 2) : 0u, rc2 = (unsigned)(Rc - 2);
+    // t = k + 1/2 as a float counter (exact: k < 2^23)
+    const float tK = (float)R.K;
+    float t = (float)k + 0.5f;
+#pragma unroll 1
+    while (t < tK) {
+        const float fi = fmaf(t, R.s[0], R.b[0]), fj = fmaf(t, R.s[1], R.b[1]),
+                    fk = fmaf(t, R.s[2], R.b[2]);
+        const int ia = (int)floorf(AXX ? fi : fk);
+        if (ia < lo || ia > hi) break;
+        // the sample's cell in the staged box: along A it starts in the slab
+        // (the test above); along y and C the box bounds come from FP32
+        // t-ranges with a one-voxel margin, so a ray that is almost
+        // perpendicular to A could still leave them -- such a sample takes
+        // trilinear0 from memory (the same arithmetic, in order)
+        const float i0f = floorf(fi), j0f = floorf(fj), k0f = floorf(fk);
+        const int i0 = (int)i0f, j0 = (int)j0f, k0 = (int)k0f;
+        const unsigned dy = (unsigned)(j0 - yl), dc = (unsigned)((AXX ? k0 : i0) - cl);
+        const bool inbox = staged && dy <= ry2 && dc <= rc2;
+        // trilinear0's arithmetic on the staged box (zero outside the
+        // volume); the cell's 8 taps lie inside the S x Ry x Rc box.  Lanes
+        // outside it read the box's first cell instead, and their value is
+        // replaced by trilinear0's
+        const float wi = fi - i0f, wj = fj - j0f, wk = fk - k0f;
+        const unsigned ab_in = __umul24((unsigned)(ia - lo), sb_a) + __umul24(dy, sj) + dc;
+        const float *p = box + (inbox ? ab_in : 0u);
+        float t000, t100, t010, t110, t001, t101, t011, t111;  // t_ijk, offsets from the cell
+        if constexpr (AXX) {  // C = z: unit stride in k, sb_a in i
+            t000 = p[0], t001 = p[1], t010 = p[sj], t011 = p[sj + 1];
+            t100 = p[sb_a], t101 = p[sb_a + 1], t110 = p[sb_a + sj], t111 = p[sb_a + sj + 1];
+        } else {  // C = x: unit stride in i, sb_a in k
+            t000 = p[0], t100 = p[1], t010 = p[sj], t110 = p[sj + 1];
+            t001 = p[sb_a], t101 = p[sb_a + 1], t011 = p[sb_a + sj], t111 = p[sb_a + sj + 1];
+        }
+        const float c00 = fmaf(wi, t100 - t000, t000);
+        const float c01 = fmaf(wi, t110 - t010, t010);
+        const float c10 = fmaf(wi, t101 - t001, t001);
+        const float c11 = fmaf(wi, t111 - t011, t011);
+        const float c0 = fmaf(wj, c01 - c00, c00);
+        const float c1 = fmaf(wj, c11 - c10, c10);
+        float v = fmaf(wk, c1 - c0, c0);
+        if (!inbox) v = trilinear0_call(J.vol, J.X, J.Y, J.Z, fi, fj, fk);
+        acc += v;
+        t += 1.0f;
+    }
+    k = (int)t;  // t = k + 1/2: truncation gives k
+}
 
 template <bool kBuf>
-__global__ __launch_bounds__(kSlabThreads, kSlabTile >= 32 ? 8 : 1) void forward_slab_kernel(
-    ForwardJob J)
+__global__ __launch_bounds__(kSlabThreads) void forward_slab_kernel(ForwardJob J,
+                                                                    const float *volT)
 {
     __shared__ float box[kSlabCap];
     // per-wave partial bounds, double-buffered by slab parity: a slab that no
     // ray samples skips its second barrier, so a wave may write the next
     // slab's partials while another still reads this slab's
     __shared__ int red[2][kSlabWaves][4];
-    const int tu = (J.nu + kSlabTile - 1) / kSlabTile;
-    const int tv = (J.nv + kSlabTile - 1) / kSlabTile;
+    const int tu = (J.nu + kSlabTU - 1) / kSlabTU;
+    const int tv = (J.nv + kSlabTV - 1) / kSlabTV;
     const int P = tu * tv;
-    int n, tile;
-    if (CTR_FWD_SLAB_XCD) {
-        // blocks reach the 8 XCDs round-robin: XCD x takes tiles x P8 ..
-        // x P8 + P8 - 1 of every view, all XCDs on the same view at a time
-        // (the grid holds 8 P8 blocks per view; the padding ones return)
-        const int P8 = (P + 7) / 8;
-        const int x = blockIdx.x & 7, kk = blockIdx.x >> 3;
-        n = kk / P8;
-        tile = x * P8 + kk % P8;
-        if (tile >= P) return;
-    } else {
-        n = blockIdx.x / P;
-        tile = blockIdx.x % P;
-    }
-    const int iu = (tile % tu) * kSlabTile + (int)(threadIdx.x % kSlabTile);
-    const int iv = (tile / tu) * kSlabTile + (int)(threadIdx.x / kSlabTile);
+    const int n = blockIdx.x / P, tile = blockIdx.x % P;
+    const int iu = (tile % tu) * kSlabTU + (int)(threadIdx.x % kSlabTU);
+    const int iv = (tile / tu) * kSlabTV + (int)(threadIdx.x / kSlabTU);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const bool pix = iu < J.nu && iv < J.nv;
     IndexRay R;
@@ -409,17 +490,25 @@ __global__ __launch_bounds__(kSlabThreads, kSlabTile >= 32 ? 8 : 1) void forward
     const double th = 2.0 * M_PI * J.rot * (double)n / (double)J.n_views + J.offset;
     double sn, cs;
     sincos(th, &sn, &cs);
-    const bool ax_x = fabs(cs) >= fabs(sn);
+    // (without the copy every view marches along z: slower for views near
+    // the x axis, the same sums)
+    const bool ax_x = volT != nullptr && fabs(cs) >= fabs(sn);
     const bool up = ax_x ? cs < 0.0 : sn < 0.0;
-    const int dimA = ax_x ? J.X : J.Z;
+    const int dimA = ax_x ? J.X : J.Z, dimC = ax_x ? J.Z : J.X;
     const float bA = ax_x ? R.b[0] : R.b[2], sA = ax_x ? R.s[0] : R.s[2];
     const float bC = ax_x ? R.b[2] : R.b[0], sC = ax_x ? R.s[2] : R.s[0];
     const bool along = up ? sA > 0.0f : sA < 0.0f;  // else: all samples after the slabs
+    // 1 / sA for the per-slab sample ranges: they only size the staged box
+    // (a sample outside it takes trilinear0), so a rounded reciprocal will do
+    const float inv_sA = along ? __builtin_amdgcn_rcpf(sA) : 0.0f;
 
-    // box fills through a buffer resource below 4 GiB, 64-bit addresses above
+    // box rows come from (a Y + y) dimC + c of the volume (A = z) or of its
+    // z-fastest copy (A = x): through a buffer resource below 4 GiB, 64-bit
+    // addresses above
+    const float *src = ax_x ? volT : J.vol;
     const unsigned vol_bytes = kBuf ? (unsigned)J.X * (unsigned)J.Y * (unsigned)J.Z * 4u : 0u;
     const __amdgpu_buffer_rsrc_t vrs = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<float *>(J.vol), (short)0, (int)vol_bytes, 0x00020000);
+        const_cast<float *>(src), (short)0, (int)vol_bytes, 0x00020000);
 
     // cells start at index -2 .. dimA along A (samples lie in the box grown
     // by one voxel); slab m holds cell starts lo .. lo + S - 2, slices lo ..
@@ -436,7 +525,7 @@ __global__ __launch_bounds__(kSlabThreads, kSlabTile >= 32 ? 8 : 1) void forward
         // their y and C cells, widened by one voxel for rounding
         int yl = 1 << 30, yh = -(1 << 30), cl = 1 << 30, ch = -(1 << 30);
         if (along && k < R.K) {
-            float ta = ((float)lo - bA) / sA, tb = ((float)(hi + 1) - bA) / sA;
+            float ta = ((float)lo - bA) * inv_sA, tb = ((float)(hi + 1) - bA) * inv_sA;
             if (ta > tb) {
                 const float t = ta;
                 ta = tb;
@@ -479,63 +568,53 @@ __global__ __launch_bounds__(kSlabThreads, kSlabTile >= 32 ? 8 : 1) void forward
         if (yh < yl) continue;  // no ray samples this slab (block-uniform)
         const int Ry = yh - yl + 1, Rc = ch - cl + 1;
         const bool staged = (long long)S * Ry * Rc <= kSlabCap;
-        // box layout, x fastest: A = x: [C][y][A], A = z: [A][y][C]
-        const int sy = ax_x ? S : Rc, sz = ax_x ? S * Ry : Rc * Ry;
-        const int x0 = ax_x ? lo : cl, z0 = ax_x ? cl : lo;
-        const int nx = ax_x ? S : Rc;
         if (staged) {
+            // element e = (a' Ry + y') Rc + c' of the box, the thread's
+            // (a', y', c') stepped by kSlabThreads elements at a time
             const int total = S * Ry * Rc;
+            const int q = kSlabThreads / Rc, qc = kSlabThreads - q * Rc;
+            const int qa = q / Ry, qy = q - qa * Ry;
+            const int r0 = (int)threadIdx.x / Rc;
+            int c = (int)threadIdx.x - r0 * Rc, a = r0 / Ry;
+            int y = r0 - a * Ry;
             for (int e = threadIdx.x; e < total; e += kSlabThreads) {
-                // e = (z' Ry + y') nx + x'
-                const int xr = e % nx, r = e / nx;
-                const int yr = r % Ry, zr = r / Ry;
-                const int x = x0 + xr, y = yl + yr, z = z0 + zr;
-                float v = 0.0f;
-                if (x >= 0 && x < J.X && y >= 0 && y < J.Y && z >= 0 && z < J.Z) {
-                    if constexpr (kBuf)
-                        v = __builtin_bit_cast(
-                            float, __builtin_amdgcn_raw_buffer_load_b32(
-                                       vrs, (unsigned)((z * J.Y + y) * J.X + x) * 4u, 0, 0));
-                    else
-                        v = J.vol[((size_t)z * J.Y + y) * J.X + x];
+                const int ga = lo + a, gy = yl + y, gc = cl + c;
+                const bool in = (unsigned)ga < (unsigned)dimA && (unsigned)gy < (unsigned)J.Y &&
+                                (unsigned)gc < (unsigned)dimC;
+                if constexpr (kBuf) {
+                    // LDS DMA: the wave's lanes stage 64 consecutive box
+                    // elements from e - lane on; a voxel outside the volume
+                    // reads zero through an offset past the buffer's end
+                    const unsigned voff =
+                        in ? (unsigned)((ga * J.Y + gy) * dimC + gc) * 4u : 0xfffffffcu;
+                    const int ebase = __builtin_amdgcn_readfirstlane(e - lane);
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                        vrs, (__attribute__((address_space(3))) void *)(box + ebase), 4, voff, 0,
+                        0, 0);
+                } else {
+                    box[e] = in ? src[((size_t)ga * J.Y + gy) * dimC + gc] : 0.0f;
                 }
-                box[e] = v;
+                c += qc;
+                int carry = 0;
+                if (c >= Rc) {
+                    c -= Rc;
+                    carry = 1;
+                }
+                y += qy + carry;
+                a += qa;
+                if (y >= Ry) {
+                    y -= Ry;
+                    ++a;
+                }
             }
+            if constexpr (kBuf) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         }
         __syncthreads();
         if (!along) continue;
-#pragma unroll 1
-        while (k < R.K) {
-            const float t = (float)k + 0.5f;
-            const float fi = fmaf(t, R.s[0], R.b[0]), fj = fmaf(t, R.s[1], R.b[1]),
-                        fk = fmaf(t, R.s[2], R.b[2]);
-            const int ia = (int)floorf(ax_x ? fi : fk);
-            if (ia < lo || ia > hi) break;
-            // the sample's cell in the staged box: along A it starts in the
-            // slab (the test above); along y and C the box bounds come from
-            // FP32 t-ranges with a one-voxel margin, so a ray that is almost
-            // perpendicular to A could still leave them -- such a sample
-            // takes trilinear0 from memory (the same arithmetic, in order)
-            const float i0f = floorf(fi), j0f = floorf(fj), k0f = floorf(fk);
-            const int i0 = (int)i0f, j0 = (int)j0f, k0 = (int)k0f;
-            const int dy = j0 - yl, dc = (ax_x ? k0 : i0) - cl;
-            if (staged && dy >= 0 && dy <= Ry - 2 && dc >= 0 && dc <= Rc - 2) {
-                // trilinear0 on the staged box (zero outside the volume); the
-                // cell's 8 taps lie inside the S x Ry x Rc box
-                const float wi = fi - i0f, wj = fj - j0f, wk = fk - k0f;
-                const float *p = box + ((i0 - x0) + dy * sy + (k0 - z0) * sz);
-                const float c00 = fmaf(wi, p[1] - p[0], p[0]);
-                const float c01 = fmaf(wi, p[sy + 1] - p[sy], p[sy]);
-                const float c10 = fmaf(wi, p[sz + 1] - p[sz], p[sz]);
-                const float c11 = fmaf(wi, p[sz + sy + 1] - p[sz + sy], p[sz + sy]);
-                const float c0 = fmaf(wj, c01 - c00, c00);
-                const float c1 = fmaf(wj, c11 - c10, c10);
-                acc += fmaf(wk, c1 - c0, c0);
-            } else {
-                acc += trilinear0(J.vol, J.X, J.Y, J.Z, fi, fj, fk);
-            }
-            ++k;
-        }
+        if (ax_x)
+            slab_samples<true>(J, R, k, acc, box, staged, lo, hi, yl, cl, Ry, Rc);
+        else
+            slab_samples<false>(J, R, k, acc, box, staged, lo, hi, yl, cl, Ry, Rc);
     }
     for (; k < R.K; ++k) {
         const float t = (float)k + 0.5f;
@@ -977,17 +1056,37 @@ int ctr_forward_project(void *stream, ctr_workspace *ws, int geometry, const flo
     // buffer resource or 64-bit addresses (CTR_OPTION_FORWARD_KERNEL =
     // CTR_KERNEL_RAY forces the ray kernel: diagnostics and tests)
     const bool slab = workspace_option(w, CTR_OPTION_FORWARD_KERNEL) != CTR_KERNEL_RAY;
-    const long long tiles = (long long)((nu + kSlabTile - 1) / kSlabTile) *
-                            ((nv + kSlabTile - 1) / kSlabTile);
-    const long long slab_blocks =
-        CTR_FWD_SLAB_XCD ? 8 * ((tiles + 7) / 8) * n_views : tiles * n_views;
+    const long long tiles = (long long)((nu + kSlabTU - 1) / kSlabTU) *
+                            ((nv + kSlabTV - 1) / kSlabTV);
+    const long long slab_blocks = tiles * n_views;
     CTR_REQUIRE(slab_blocks < (1LL << 31), CTR_ERR_INVALID_ARGUMENT,
                 "forward projection too large");
     const dim3 slab_grid((unsigned)slab_blocks);
+    const float *volT = nullptr;
+    if (slab) {
+        // the z-fastest copy the slab kernel reads for views marching along
+        // x (|cos theta| >= |sin theta| on the device; the margin covers host
+        // and device rounding), rebuilt on every call
+        bool any_x = false;
+        for (int n = 0; n < n_views && !any_x; ++n) {
+            const double th = 2.0 * M_PI * rotation_count * (double)n / (double)n_views +
+                              rotation_offset;
+            any_x = fabs(cos(th)) >= fabs(sin(th)) - 1e-6;
+        }
+        if (any_x) {
+            void *t = nullptr;
+            if ((rc = workspace_get(w, SLOT_VOLUME_T, bytes, &t))) return rc;
+            const dim3 tg((unsigned)(((X + 31) / 32) * ((Z + 31) / 32)),
+                          (unsigned)(Y < 65535 ? Y : 65535));
+            transpose_xz_kernel<<<tg, 256, 0, as_stream(stream)>>>(volume, (float *)t, X, Y, Z);
+            CTR_CHECK_LAUNCH();
+            volT = (const float *)t;
+        }
+    }
     if (slab && buf)
-        forward_slab_kernel<true><<<slab_grid, kSlabThreads, 0, as_stream(stream)>>>(J);
+        forward_slab_kernel<true><<<slab_grid, kSlabThreads, 0, as_stream(stream)>>>(J, volT);
     else if (slab)
-        forward_slab_kernel<false><<<slab_grid, kSlabThreads, 0, as_stream(stream)>>>(J);
+        forward_slab_kernel<false><<<slab_grid, kSlabThreads, 0, as_stream(stream)>>>(J, volT);
     else if (buf)
         forward_kernel<true><<<grid, 256, 0, as_stream(stream)>>>(J);
     else
