@@ -174,16 +174,16 @@ struct IndexRay {
     int K;
 };
 
-__device__ inline IndexRay index_ray(const ForwardJob &J, int n, int iu, int iv)
+// World-space line of detector pixel (iu, iv) in view n: origin o and unit
+// direction dir (cone: from the source).
+__device__ inline void pixel_line(const ForwardJob &J, int n, int iu, int iv, double o[3],
+                                  double dir[3])
 {
-    IndexRay R;
-    R.K = 0;
     double th = 2.0 * M_PI * J.rot * (double)n / (double)J.n_views + J.offset;
     double s, c;
     sincos(th, &s, &c);
     double u = (iu + 0.5 - 0.5 * J.nu) * J.delta;
     double v = (iv + 0.5 - 0.5 * J.nv) * J.delta;
-    double o[3], dir[3];
     if (J.geometry == CTR_GEOMETRY_CONE) {
         o[0] = J.sod * c;
         o[1] = 0.0;
@@ -203,6 +203,29 @@ __device__ inline IndexRay index_ray(const ForwardJob &J, int n, int iu, int iv)
         dir[1] = 0.0;
         dir[2] = -s;
     }
+}
+
+// The same line in voxel-index coordinates, FP32: L[0..2] a point, L[3..5]
+// the direction (per unit world length).
+__device__ inline void index_line(const ForwardJob &J, int n, int iu, int iv, float L[6])
+{
+    double o[3], dir[3];
+    pixel_line(J, n, iu, iv, o, dir);
+    const double inv_d = 1.0 / J.d;
+    L[0] = (float)(o[0] * inv_d + 0.5 * J.X - 0.5);
+    L[1] = (float)(o[1] * inv_d + 0.5 * J.Y - 0.5);
+    L[2] = (float)(o[2] * inv_d + 0.5 * J.Z - 0.5);
+    L[3] = (float)(dir[0] * inv_d);
+    L[4] = (float)(dir[1] * inv_d);
+    L[5] = (float)(dir[2] * inv_d);
+}
+
+__device__ inline IndexRay index_ray(const ForwardJob &J, int n, int iu, int iv)
+{
+    IndexRay R;
+    R.K = 0;
+    double o[3], dir[3];
+    pixel_line(J, n, iu, iv, o, dir);
     const double half[3] = {(0.5 * J.X + 1.0) * J.d, (0.5 * J.Y + 1.0) * J.d,
                             (0.5 * J.Z + 1.0) * J.d};
     double tmin = -1e300, tmax = 1e300;
@@ -338,22 +361,54 @@ __global__ __launch_bounds__(256) void forward_kernel(ForwardJob J)
 // ---------------------------------------------------------------------------
 // Slab depth, box capacity (floats) and the detector tile (u x v rays).
 // Measured (ms; 512^3 x 720 views of 768^2, 256^3 x 360 of 384^2, 1440^3 x
-// 96 of 1536^2; all bit-identical): 16 x 16 rays, 6 slices, 18 KB 583 / 35.2
-// / 1242; 32 x 8 623 / 37.4 / 1351; 64 x 4 845 / 46.0 / 1871; 32 x 8 with 8
-// slices in 24 KB 641 / 37.4 / 1420.  The earlier fill (box x fastest, read
-// row by row along x for every view, element indices by two integer
-// divisions) measured 667 / 41.2 / 1393 at 16 x 16.  Before it: 4 slices /
-// 12 KB 837 ms at 512^3, 5 / 15 KB 696, 8 / 20 KB 676, 12 / 36 KB 831,
-// 16 / 48 KB 1158; 32 x 32 rays with 36-48 KB boxes 756-803.
+// 96 of 1536^2; all bit-identical), this kernel: 8 slices / 24 KB 347 / 22.5
+// / 1138; 6 / 18 KB 378 / 24.7 / 1161; 7 / 21 KB 360 / 23.5 / 1144; 8 /
+// 20 KB 365 / 22.7 / 1202; 8 / 28 KB 377 / 24.4 / 1220; 9 / 27 KB 370 / 24.0
+// / 1302; 10 / 30 KB 367 / 23.5 / 1318; 12 / 36 KB 412 / 26.2 / 1586; 32 x 8
+// rays, 8 / 24 KB 393 / 24.1 / 1254.  With the earlier per-ray box bounds
+// (wave reductions) and plain loads: 16 x 16, 6 / 18 KB 434 / 27.9 / 1144;
+// 32 x 8 623, 64 x 4 845 (512^3, before the DMA fill).  The first slab
+// kernel (box x fastest, read along x for every view, element indices by
+// integer division) measured 667 / 41.2 / 1393.
 #ifndef CTR_FWD_SLAB_S
-#define CTR_FWD_SLAB_S 6
+#define CTR_FWD_SLAB_S 8
 #endif
 #ifndef CTR_FWD_SLAB_CAP
-#define CTR_FWD_SLAB_CAP 4608
+#define CTR_FWD_SLAB_CAP 6144
 #endif
 #ifndef CTR_FWD_SLAB_TU
 #define CTR_FWD_SLAB_TU 16
 #endif
+// Tile order over the XCDs (workgroups reach the 8 XCDs round-robin by
+// index): 0 plain (consecutive tiles on different XCDs); 1 each XCD takes one
+// contiguous eighth of every view's tiles; 2 groups of G x G neighbouring
+// tiles are dealt to the XCDs round-robin, a group's tiles consecutive on its
+// XCD (neighbours' overlapping boxes meet in one L2, every XCD still covers
+// the whole detector).  Measured (ms, the three inputs of the table above):
+// 0: 347 / 22.4 / 1135; 1: 378 / 24.6 / 1065 (the XCDs holding the
+// detector's edge rows run out of work early); 2 with G = 2: 344 / 22.2 /
+// 1057, G = 3: 360 / 26.4 / 1054, G = 4: 347 / 24.5 / 1044.
+#ifndef CTR_FWD_SLAB_XCD
+#define CTR_FWD_SLAB_XCD 2
+#endif
+#ifndef CTR_FWD_SLAB_GROUP
+#define CTR_FWD_SLAB_GROUP 2
+#endif
+constexpr int kSlabGroup = CTR_FWD_SLAB_GROUP;
+
+// Blocks per view of the slab grid and the (view, tile) of block b; false
+// for the grid's padding blocks.
+__host__ __device__ inline long long slab_blocks_per_view(int tu, int tv)
+{
+    const long long P = (long long)tu * tv;
+    if (CTR_FWD_SLAB_XCD == 1) return 8 * ((P + 7) / 8);
+    if (CTR_FWD_SLAB_XCD == 2) {
+        const long long groups =
+            (long long)((tu + kSlabGroup - 1) / kSlabGroup) * ((tv + kSlabGroup - 1) / kSlabGroup);
+        return 8 * ((groups + 7) / 8) * kSlabGroup * kSlabGroup;
+    }
+    return P;
+}
 constexpr int kSlabS = CTR_FWD_SLAB_S;      // slices per slab
 constexpr int kSlabCap = CTR_FWD_SLAB_CAP;  // floats of the staged box
 constexpr int kSlabThreads = 256, kSlabWaves = kSlabThreads / 64;
@@ -368,6 +423,22 @@ __device__ __attribute__((noinline)) float trilinear0_call(const float *__restri
                                                             float fk)
 {
     return trilinear0(vol, X, Y, Z, fi, fj, fk);
+}
+
+// q = x / d, r = x % d for 0 <= x < 2^16 and d > 0 (here x < 256): a float
+// estimate of the quotient, off by at most one there even with the
+// reciprocal one ulp off, and one correction step.
+__device__ inline void small_divmod(int x, int d, int &q, int &r)
+{
+    q = (int)((float)x * __builtin_amdgcn_rcpf((float)d));
+    r = x - q * d;
+    if (r < 0) {
+        --q;
+        r += d;
+    } else if (r >= d) {
+        ++q;
+        r -= d;
+    }
 }
 
 __device__ inline int wave_min(int v)
@@ -469,14 +540,41 @@ __global__ __launch_bounds__(kSlabThreads) void forward_slab_kernel(ForwardJob J
                                                                     const float *volT)
 {
     __shared__ float box[kSlabCap];
-    // per-wave partial bounds, double-buffered by slab parity: a slab that no
-    // ray samples skips its second barrier, so a wave may write the next
-    // slab's partials while another still reads this slab's
-    __shared__ int red[2][kSlabWaves][4];
+    __shared__ int red[kSlabWaves][2];  // per-wave sample range along A
+    __shared__ float4 corner[4];        // the tile's corner rays: y, C = (Y0, dY, C0, dC) . (1, A)
     const int tu = (J.nu + kSlabTU - 1) / kSlabTU;
     const int tv = (J.nv + kSlabTV - 1) / kSlabTV;
     const int P = tu * tv;
-    const int n = blockIdx.x / P, tile = blockIdx.x % P;
+    int n, tile;
+    if (CTR_FWD_SLAB_XCD == 1) {
+        // the grid holds 8 P8 blocks per view: XCD x = index mod 8 takes
+        // tiles x P8 .. x P8 + P8 - 1 of each view in order; the padding
+        // blocks return at once (no barrier reached)
+        const int P8 = (P + 7) / 8;
+        const int x = (int)(blockIdx.x & 7), kk = (int)(blockIdx.x >> 3);
+        n = kk / P8;
+        tile = x * P8 + kk % P8;
+        if (tile >= P) return;
+    } else if (CTR_FWD_SLAB_XCD == 2) {
+        // block b: XCD x = b mod 8, its j-th block j = b / 8 = (view, group
+        // slot, tile in group); group g = slot * 8 + x of the view's G x G
+        // groups (row-major over the tile grid); padding blocks return
+        constexpr int G2 = kSlabGroup * kSlabGroup;
+        const int gu = (tu + kSlabGroup - 1) / kSlabGroup;
+        const int groups = gu * ((tv + kSlabGroup - 1) / kSlabGroup);
+        const int per_x = (groups + 7) / 8;  // group slots per XCD and view
+        const int x = (int)(blockIdx.x & 7), j = (int)(blockIdx.x >> 3);
+        n = j / (per_x * G2);
+        const int rem = j - n * per_x * G2;
+        const int g = (rem / G2) * 8 + x, t = rem % G2;
+        const int u = (g % gu) * kSlabGroup + t % kSlabGroup;
+        const int v = (g / gu) * kSlabGroup + t / kSlabGroup;
+        if (g >= groups || u >= tu || v >= tv) return;
+        tile = v * tu + u;
+    } else {
+        n = blockIdx.x / P;
+        tile = blockIdx.x % P;
+    }
     const int iu = (tile % tu) * kSlabTU + (int)(threadIdx.x % kSlabTU);
     const int iv = (tile / tu) * kSlabTV + (int)(threadIdx.x / kSlabTU);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -496,11 +594,47 @@ __global__ __launch_bounds__(kSlabThreads) void forward_slab_kernel(ForwardJob J
     const bool up = ax_x ? cs < 0.0 : sn < 0.0;
     const int dimA = ax_x ? J.X : J.Z, dimC = ax_x ? J.Z : J.X;
     const float bA = ax_x ? R.b[0] : R.b[2], sA = ax_x ? R.s[0] : R.s[2];
-    const float bC = ax_x ? R.b[2] : R.b[0], sC = ax_x ? R.s[2] : R.s[0];
     const bool along = up ? sA > 0.0f : sA < 0.0f;  // else: all samples after the slabs
-    // 1 / sA for the per-slab sample ranges: they only size the staged box
-    // (a sample outside it takes trilinear0), so a rounded reciprocal will do
-    const float inv_sA = along ? __builtin_amdgcn_rcpf(sA) : 0.0f;
+
+    // once per block: the cells along A of the along-rays' first and last
+    // samples, and the tile's four corner rays.  Every ray of the tile lies
+    // in the convex hull of the corner rays, so between two planes of
+    // constant A its points lie in the bounding box of the corner rays' 8
+    // points there: the staged box of a slab (a sample outside it still
+    // takes trilinear0, so the bounds only size the box)
+    {
+        int amin = 1 << 30, amax = -(1 << 30);
+        if (along && R.K > 0) {
+            const float f0 = fmaf(0.5f, sA, bA), f1 = fmaf((float)R.K - 0.5f, sA, bA);
+            amin = (int)floorf(fminf(f0, f1));
+            amax = (int)floorf(fmaxf(f0, f1));
+        }
+        amin = wave_min(amin);
+        amax = wave_max(amax);
+        if (lane == 0) {
+            red[wave][0] = amin;
+            red[wave][1] = amax;
+        }
+        if (threadIdx.x < 4) {
+            const int u0 = (tile % tu) * kSlabTU, v0 = (tile / tu) * kSlabTV;
+            const int cu = min(u0 + (int)(threadIdx.x & 1) * (kSlabTU - 1), J.nu - 1);
+            const int cv = min(v0 + (int)(threadIdx.x >> 1) * (kSlabTV - 1), J.nv - 1);
+            float L[6];
+            index_line(J, n, cu, cv, L);
+            // y and C of the ray as functions of A: y = Y0 + A dY, C = C0 + A dC
+            const float cA = ax_x ? L[0] : L[2], dA = ax_x ? L[3] : L[5];
+            const float dY = L[4] / dA, dC = (ax_x ? L[5] : L[3]) / dA;
+            corner[threadIdx.x] = make_float4(fmaf(-cA, dY, L[1]), dY,
+                                              fmaf(-cA, dC, ax_x ? L[2] : L[0]), dC);
+        }
+    }
+    __syncthreads();
+    int amin = red[0][0], amax = red[0][1];
+#pragma unroll
+    for (int w = 1; w < kSlabWaves; ++w) {
+        amin = min(amin, red[w][0]);
+        amax = max(amax, red[w][1]);
+    }
 
     // box rows come from (a Y + y) dimC + c of the volume (A = z) or of its
     // z-fastest copy (A = x): through a buffer resource below 4 GiB, 64-bit
@@ -521,62 +655,50 @@ __global__ __launch_bounds__(kSlabThreads) void forward_slab_kernel(ForwardJob J
     for (int mm = 0; mm < M; ++mm) {
         const int m = up ? mm : M - 1 - mm;
         const int lo = -2 + m * (S - 1), hi = lo + S - 2;
-        // this ray's samples in the slab: t = k + 1/2 with lo <= f_A(t) < hi + 1;
-        // their y and C cells, widened by one voxel for rounding
-        int yl = 1 << 30, yh = -(1 << 30), cl = 1 << 30, ch = -(1 << 30);
-        if (along && k < R.K) {
-            float ta = ((float)lo - bA) * inv_sA, tb = ((float)(hi + 1) - bA) * inv_sA;
-            if (ta > tb) {
-                const float t = ta;
-                ta = tb;
-                tb = t;
-            }
-            ta = fmaxf(ta, (float)k + 0.5f) - 0.5f;
-            tb = fminf(tb, (float)R.K - 0.5f) + 0.5f;
-            if (ta <= tb) {
-                const float y0 = fmaf(ta, R.s[1], R.b[1]), y1 = fmaf(tb, R.s[1], R.b[1]);
-                const float c0 = fmaf(ta, sC, bC), c1 = fmaf(tb, sC, bC);
-                yl = (int)floorf(fminf(y0, y1)) - 1;
-                yh = (int)floorf(fmaxf(y0, y1)) + 2;
-                cl = (int)floorf(fminf(c0, c1)) - 1;
-                ch = (int)floorf(fmaxf(c0, c1)) + 2;
-            }
-        }
-        yl = wave_min(yl);
-        yh = wave_max(yh);
-        cl = wave_min(cl);
-        ch = wave_max(ch);
-        int(*rd)[4] = red[mm & 1];
-        if (lane == 0) {
-            rd[wave][0] = yl;
-            rd[wave][1] = yh;
-            rd[wave][2] = cl;
-            rd[wave][3] = ch;
-        }
-        __syncthreads();  // also: every ray is done with the previous slab's box
-        yl = rd[0][0];
-        yh = rd[0][1];
-        cl = rd[0][2];
-        ch = rd[0][3];
+        if (hi < amin || lo > amax) continue;  // no along-ray sample starts a cell here
+        __syncthreads();  // every ray is done with the previous slab's box
+        // the corner rays' y and C on the planes A = lo and A = hi + 1 (the
+        // slab's cells start in [lo, hi + 1)), widened by one voxel for
+        // rounding and clamped to the cells samples can have (-2 .. n); the
+        // lines are read from LDS after the barrier, not kept in registers
+        float ylo = 3.0e38f, yhi = -3.0e38f, clo = 3.0e38f, chi = -3.0e38f;
 #pragma unroll
-        for (int w = 1; w < kSlabWaves; ++w) {
-            yl = min(yl, rd[w][0]);
-            yh = max(yh, rd[w][1]);
-            cl = min(cl, rd[w][2]);
-            ch = max(ch, rd[w][3]);
+        for (int q = 0; q < 4; ++q) {
+            const float4 L = corner[q];
+#pragma unroll
+            for (int f = 0; f < 2; ++f) {
+                const float pa = (float)(f ? hi + 1 : lo);
+                const float y = fmaf(pa, L.y, L.x), c = fmaf(pa, L.w, L.z);
+                ylo = fminf(ylo, y);
+                yhi = fmaxf(yhi, y);
+                clo = fminf(clo, c);
+                chi = fmaxf(chi, c);
+            }
         }
-        if (yh < yl) continue;  // no ray samples this slab (block-uniform)
+        // a corner ray (almost) parallel to the planes gives no finite bound
+        if (!(ylo <= yhi && clo <= chi)) {
+            ylo = -4.0f;
+            yhi = (float)J.Y + 4.0f;
+            clo = -4.0f;
+            chi = (float)dimC + 4.0f;
+        }
+        const float ymax = (float)J.Y + 4.0f, cmax = (float)dimC + 4.0f;
+        const int yl = max((int)floorf(fminf(fmaxf(ylo, -4.0f), ymax)) - 1, -2);
+        const int yh = min((int)floorf(fminf(fmaxf(yhi, -4.0f), ymax)) + 2, J.Y + 1);
+        const int cl = max((int)floorf(fminf(fmaxf(clo, -4.0f), cmax)) - 1, -2);
+        const int ch = min((int)floorf(fminf(fmaxf(chi, -4.0f), cmax)) + 2, dimC + 1);
+        if (yh < yl || ch < cl) continue;  // block-uniform
         const int Ry = yh - yl + 1, Rc = ch - cl + 1;
         const bool staged = (long long)S * Ry * Rc <= kSlabCap;
         if (staged) {
             // element e = (a' Ry + y') Rc + c' of the box, the thread's
             // (a', y', c') stepped by kSlabThreads elements at a time
             const int total = S * Ry * Rc;
-            const int q = kSlabThreads / Rc, qc = kSlabThreads - q * Rc;
-            const int qa = q / Ry, qy = q - qa * Ry;
-            const int r0 = (int)threadIdx.x / Rc;
-            int c = (int)threadIdx.x - r0 * Rc, a = r0 / Ry;
-            int y = r0 - a * Ry;
+            int q, qc, qa, qy, r0, c, a, y;
+            small_divmod(kSlabThreads, Rc, q, qc);
+            small_divmod(q, Ry, qa, qy);
+            small_divmod((int)threadIdx.x, Rc, r0, c);
+            small_divmod(r0, Ry, a, y);
             for (int e = threadIdx.x; e < total; e += kSlabThreads) {
                 const int ga = lo + a, gy = yl + y, gc = cl + c;
                 const bool in = (unsigned)ga < (unsigned)dimA && (unsigned)gy < (unsigned)J.Y &&
@@ -1056,9 +1178,9 @@ int ctr_forward_project(void *stream, ctr_workspace *ws, int geometry, const flo
     // buffer resource or 64-bit addresses (CTR_OPTION_FORWARD_KERNEL =
     // CTR_KERNEL_RAY forces the ray kernel: diagnostics and tests)
     const bool slab = workspace_option(w, CTR_OPTION_FORWARD_KERNEL) != CTR_KERNEL_RAY;
-    const long long tiles = (long long)((nu + kSlabTU - 1) / kSlabTU) *
-                            ((nv + kSlabTV - 1) / kSlabTV);
-    const long long slab_blocks = tiles * n_views;
+    const long long slab_blocks =
+        slab_blocks_per_view((nu + kSlabTU - 1) / kSlabTU, (nv + kSlabTV - 1) / kSlabTV) *
+        n_views;
     CTR_REQUIRE(slab_blocks < (1LL << 31), CTR_ERR_INVALID_ARGUMENT,
                 "forward projection too large");
     const dim3 slab_grid((unsigned)slab_blocks);
