@@ -395,9 +395,16 @@ __global__ __launch_bounds__(256) void forward_kernel(ForwardJob J)
 #define CTR_FWD_SLAB_GROUP 2
 #endif
 constexpr int kSlabGroup = CTR_FWD_SLAB_GROUP;
+// Consecutive views per block (1 or 2, chosen per call from the geometry):
+// their rays through one detector tile cross nearly the same voxels, so one
+// staged box (bounded by all their corner rays) serves them all.  A voxel r
+// voxels from the rotation axis moves about r dtheta between views; the box
+// grows by that much per extra view.  CTR_FWD_SLAB_VIEWS forces a count.
+#ifndef CTR_FWD_SLAB_VIEWS
+#define CTR_FWD_SLAB_VIEWS 0
+#endif
 
-// Blocks per view of the slab grid and the (view, tile) of block b; false
-// for the grid's padding blocks.
+// Blocks per view group of the slab grid (padding blocks included).
 __host__ __device__ inline long long slab_blocks_per_view(int tu, int tv)
 {
     const long long P = (long long)tu * tv;
@@ -424,6 +431,10 @@ __device__ __attribute__((noinline)) float trilinear0_call(const float *__restri
 {
     return trilinear0(vol, X, Y, Z, fi, fj, fk);
 }
+
+// A zero the box fill of volumes of 4 GiB and more reads for voxels outside
+// the volume (device globals start zeroed; nothing writes it).
+static __device__ float g_zero_word[1];
 
 // q = x / d, r = x % d for 0 <= x < 2^16 and d > 0 (here x < 256): a float
 // estimate of the quotient, off by at most one there even with the
@@ -535,57 +546,65 @@ __device__ inline void slab_samples(const ForwardJob &J, const IndexRay &R, int 
     k = (int)t;  // t = k + 1/2: truncation gives k
 }
 
-template <bool kBuf>
+template <bool kBuf, int V>
 __global__ __launch_bounds__(kSlabThreads) void forward_slab_kernel(ForwardJob J,
                                                                     const float *volT)
 {
     __shared__ float box[kSlabCap];
     __shared__ int red[kSlabWaves][2];  // per-wave sample range along A
-    __shared__ float4 corner[4];        // the tile's corner rays: y, C = (Y0, dY, C0, dC) . (1, A)
+    __shared__ float4 corner[4 * V];    // corner rays: y, C = (Y0, dY, C0, dC) . (1, A)
     const int tu = (J.nu + kSlabTU - 1) / kSlabTU;
     const int tv = (J.nv + kSlabTV - 1) / kSlabTV;
     const int P = tu * tv;
-    int n, tile;
+    int ng, tile;  // view group (views ng V .. ng V + V - 1) and tile
     if (CTR_FWD_SLAB_XCD == 1) {
-        // the grid holds 8 P8 blocks per view: XCD x = index mod 8 takes
-        // tiles x P8 .. x P8 + P8 - 1 of each view in order; the padding
-        // blocks return at once (no barrier reached)
+        // the grid holds 8 P8 blocks per view group: XCD x = index mod 8
+        // takes tiles x P8 .. x P8 + P8 - 1 of each group in order; the
+        // padding blocks return at once (no barrier reached)
         const int P8 = (P + 7) / 8;
         const int x = (int)(blockIdx.x & 7), kk = (int)(blockIdx.x >> 3);
-        n = kk / P8;
+        ng = kk / P8;
         tile = x * P8 + kk % P8;
         if (tile >= P) return;
     } else if (CTR_FWD_SLAB_XCD == 2) {
-        // block b: XCD x = b mod 8, its j-th block j = b / 8 = (view, group
-        // slot, tile in group); group g = slot * 8 + x of the view's G x G
-        // groups (row-major over the tile grid); padding blocks return
+        // block b: XCD x = b mod 8, its j-th block j = b / 8 = (view group,
+        // group slot, tile in group); tile group g = slot * 8 + x of the
+        // G x G groups (row-major over the tile grid); padding blocks return
         constexpr int G2 = kSlabGroup * kSlabGroup;
         const int gu = (tu + kSlabGroup - 1) / kSlabGroup;
         const int groups = gu * ((tv + kSlabGroup - 1) / kSlabGroup);
-        const int per_x = (groups + 7) / 8;  // group slots per XCD and view
+        const int per_x = (groups + 7) / 8;  // group slots per XCD and view group
         const int x = (int)(blockIdx.x & 7), j = (int)(blockIdx.x >> 3);
-        n = j / (per_x * G2);
-        const int rem = j - n * per_x * G2;
+        ng = j / (per_x * G2);
+        const int rem = j - ng * per_x * G2;
         const int g = (rem / G2) * 8 + x, t = rem % G2;
         const int u = (g % gu) * kSlabGroup + t % kSlabGroup;
         const int v = (g / gu) * kSlabGroup + t / kSlabGroup;
         if (g >= groups || u >= tu || v >= tv) return;
         tile = v * tu + u;
     } else {
-        n = blockIdx.x / P;
+        ng = blockIdx.x / P;
         tile = blockIdx.x % P;
     }
+    const int n0 = ng * V;
     const int iu = (tile % tu) * kSlabTU + (int)(threadIdx.x % kSlabTU);
     const int iv = (tile / tu) * kSlabTV + (int)(threadIdx.x / kSlabTU);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const bool pix = iu < J.nu && iv < J.nv;
-    IndexRay R;
-    R.K = 0;
-    for (int a = 0; a < 3; a++) R.b[a] = R.s[a] = 0.0f;
-    if (pix) R = index_ray(J, n, iu, iv);
+    // this pixel's ray in each of the block's views
+    IndexRay R[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        R[v].K = 0;
+        for (int a = 0; a < 3; a++) R[v].b[a] = R[v].s[a] = 0.0f;
+        if (pix && n0 + v < J.n_views) R[v] = index_ray(J, n0 + v, iu, iv);
+    }
 
-    // the view's main horizontal axis and the central ray's direction on it
-    const double th = 2.0 * M_PI * J.rot * (double)n / (double)J.n_views + J.offset;
+    // the main horizontal axis of the block's first view and its central
+    // ray's direction on it (the block's other views, a fraction of a degree
+    // further on, march along the same axis; a ray running against it takes
+    // its samples after the slabs)
+    const double th = 2.0 * M_PI * J.rot * (double)n0 / (double)J.n_views + J.offset;
     double sn, cs;
     sincos(th, &sn, &cs);
     // (without the copy every view marches along z: slower for views near
@@ -593,21 +612,29 @@ __global__ __launch_bounds__(kSlabThreads) void forward_slab_kernel(ForwardJob J
     const bool ax_x = volT != nullptr && fabs(cs) >= fabs(sn);
     const bool up = ax_x ? cs < 0.0 : sn < 0.0;
     const int dimA = ax_x ? J.X : J.Z, dimC = ax_x ? J.Z : J.X;
-    const float bA = ax_x ? R.b[0] : R.b[2], sA = ax_x ? R.s[0] : R.s[2];
-    const bool along = up ? sA > 0.0f : sA < 0.0f;  // else: all samples after the slabs
+    bool along[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        const float sA = ax_x ? R[v].s[0] : R[v].s[2];
+        along[v] = up ? sA > 0.0f : sA < 0.0f;  // else: all samples after the slabs
+    }
 
     // once per block: the cells along A of the along-rays' first and last
-    // samples, and the tile's four corner rays.  Every ray of the tile lies
-    // in the convex hull of the corner rays, so between two planes of
-    // constant A its points lie in the bounding box of the corner rays' 8
-    // points there: the staged box of a slab (a sample outside it still
-    // takes trilinear0, so the bounds only size the box)
+    // samples, and the tile's four corner rays in each view.  Every ray of
+    // the tile lies in the convex hull of its view's corner rays, so between
+    // two planes of constant A its points lie in the bounding box of those
+    // corner rays' points there: the staged box of a slab (a sample outside
+    // it still takes trilinear0, so the bounds only size the box)
     {
         int amin = 1 << 30, amax = -(1 << 30);
-        if (along && R.K > 0) {
-            const float f0 = fmaf(0.5f, sA, bA), f1 = fmaf((float)R.K - 0.5f, sA, bA);
-            amin = (int)floorf(fminf(f0, f1));
-            amax = (int)floorf(fmaxf(f0, f1));
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            if (along[v] && R[v].K > 0) {
+                const float bA = ax_x ? R[v].b[0] : R[v].b[2], sA = ax_x ? R[v].s[0] : R[v].s[2];
+                const float f0 = fmaf(0.5f, sA, bA), f1 = fmaf((float)R[v].K - 0.5f, sA, bA);
+                amin = min(amin, (int)floorf(fminf(f0, f1)));
+                amax = max(amax, (int)floorf(fmaxf(f0, f1)));
+            }
         }
         amin = wave_min(amin);
         amax = wave_max(amax);
@@ -615,12 +642,13 @@ __global__ __launch_bounds__(kSlabThreads) void forward_slab_kernel(ForwardJob J
             red[wave][0] = amin;
             red[wave][1] = amax;
         }
-        if (threadIdx.x < 4) {
+        if (threadIdx.x < 4 * V) {
+            const int q = threadIdx.x & 3, nv_ = min(n0 + (int)threadIdx.x / 4, J.n_views - 1);
             const int u0 = (tile % tu) * kSlabTU, v0 = (tile / tu) * kSlabTV;
-            const int cu = min(u0 + (int)(threadIdx.x & 1) * (kSlabTU - 1), J.nu - 1);
-            const int cv = min(v0 + (int)(threadIdx.x >> 1) * (kSlabTV - 1), J.nv - 1);
+            const int cu = min(u0 + (q & 1) * (kSlabTU - 1), J.nu - 1);
+            const int cv = min(v0 + (q >> 1) * (kSlabTV - 1), J.nv - 1);
             float L[6];
-            index_line(J, n, cu, cv, L);
+            index_line(J, nv_, cu, cv, L);
             // y and C of the ray as functions of A: y = Y0 + A dY, C = C0 + A dC
             const float cA = ax_x ? L[0] : L[2], dA = ax_x ? L[3] : L[5];
             const float dY = L[4] / dA, dC = (ax_x ? L[5] : L[3]) / dA;
@@ -645,48 +673,65 @@ __global__ __launch_bounds__(kSlabThreads) void forward_slab_kernel(ForwardJob J
         const_cast<float *>(src), (short)0, (int)vol_bytes, 0x00020000);
 
     // cells start at index -2 .. dimA along A (samples lie in the box grown
-    // by one voxel); slab m holds cell starts lo .. lo + S - 2, slices lo ..
-    // lo + S - 1, lo = -2 + m (S - 1)
-    constexpr int S = kSlabS;
-    const int M = (dimA + 3 + S - 2) / (S - 1);
-    float acc = 0.0f;
-    int k = 0;
+    // by one voxel).  The slabs cover the cells of the block's samples,
+    // amin .. amax, in the march direction: a slab of S slices holds cell
+    // starts lo .. lo + S - 2 and slices lo .. lo + S - 1, S = kSlabS unless
+    // its box would exceed the LDS region (wide detector pixels, views far
+    // apart): then the deepest slab that fits, down to S = 2.
+    float acc[V];
+    int k[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        acc[v] = 0.0f;
+        k[v] = 0;
+    }
+    const int cfirst = max(amin, -2), clast = min(amax, dimA);
+    int cur = up ? cfirst : clast;  // next cell start to cover
 #pragma unroll 1
-    for (int mm = 0; mm < M; ++mm) {
-        const int m = up ? mm : M - 1 - mm;
-        const int lo = -2 + m * (S - 1), hi = lo + S - 2;
-        if (hi < amin || lo > amax) continue;  // no along-ray sample starts a cell here
+    while (up ? cur <= clast : cur >= cfirst) {
         __syncthreads();  // every ray is done with the previous slab's box
-        // the corner rays' y and C on the planes A = lo and A = hi + 1 (the
-        // slab's cells start in [lo, hi + 1)), widened by one voxel for
-        // rounding and clamped to the cells samples can have (-2 .. n); the
-        // lines are read from LDS after the barrier, not kept in registers
-        float ylo = 3.0e38f, yhi = -3.0e38f, clo = 3.0e38f, chi = -3.0e38f;
+        int S = kSlabS, lo = 0, hi = 0, yl = 0, yh = -1, cl = 0, ch = -1;
+#pragma unroll 1
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            lo = up ? cur : cur - (S - 2);
+            hi = lo + S - 2;
+            // the corner rays' y and C on the planes A = lo and A = hi + 1
+            // (the slab's cells start in [lo, hi + 1)), widened by one voxel
+            // for rounding and clamped to the cells samples can have (-2 ..
+            // n); the lines are read from LDS, not kept in registers
+            float ylo = 3.0e38f, yhi = -3.0e38f, clo = 3.0e38f, chi = -3.0e38f;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const float4 L = corner[q];
+            for (int q = 0; q < 4 * V; ++q) {
+                const float4 L = corner[q];
 #pragma unroll
-            for (int f = 0; f < 2; ++f) {
-                const float pa = (float)(f ? hi + 1 : lo);
-                const float y = fmaf(pa, L.y, L.x), c = fmaf(pa, L.w, L.z);
-                ylo = fminf(ylo, y);
-                yhi = fmaxf(yhi, y);
-                clo = fminf(clo, c);
-                chi = fmaxf(chi, c);
+                for (int f = 0; f < 2; ++f) {
+                    const float pa = (float)(f ? hi + 1 : lo);
+                    const float y = fmaf(pa, L.y, L.x), c = fmaf(pa, L.w, L.z);
+                    ylo = fminf(ylo, y);
+                    yhi = fmaxf(yhi, y);
+                    clo = fminf(clo, c);
+                    chi = fmaxf(chi, c);
+                }
             }
+            // a corner ray (almost) parallel to the planes gives no finite bound
+            if (!(ylo <= yhi && clo <= chi)) {
+                ylo = -4.0f;
+                yhi = (float)J.Y + 4.0f;
+                clo = -4.0f;
+                chi = (float)dimC + 4.0f;
+            }
+            const float ymax = (float)J.Y + 4.0f, cmax = (float)dimC + 4.0f;
+            yl = max((int)floorf(fminf(fmaxf(ylo, -4.0f), ymax)) - 1, -2);
+            yh = min((int)floorf(fminf(fmaxf(yhi, -4.0f), ymax)) + 2, J.Y + 1);
+            cl = max((int)floorf(fminf(fmaxf(clo, -4.0f), cmax)) - 1, -2);
+            ch = min((int)floorf(fminf(fmaxf(chi, -4.0f), cmax)) + 2, dimC + 1);
+            const long long face = (long long)max(yh - yl + 1, 1) * max(ch - cl + 1, 1);
+            // (lo, hi and the bounds stay those of the last S tried: a box
+            // that still does not fit reads the volume directly)
+            if ((long long)S * face <= kSlabCap || S == 2 || attempt == 1) break;
+            S = (int)max(2LL, min((long long)S - 1, kSlabCap / face));
         }
-        // a corner ray (almost) parallel to the planes gives no finite bound
-        if (!(ylo <= yhi && clo <= chi)) {
-            ylo = -4.0f;
-            yhi = (float)J.Y + 4.0f;
-            clo = -4.0f;
-            chi = (float)dimC + 4.0f;
-        }
-        const float ymax = (float)J.Y + 4.0f, cmax = (float)dimC + 4.0f;
-        const int yl = max((int)floorf(fminf(fmaxf(ylo, -4.0f), ymax)) - 1, -2);
-        const int yh = min((int)floorf(fminf(fmaxf(yhi, -4.0f), ymax)) + 2, J.Y + 1);
-        const int cl = max((int)floorf(fminf(fmaxf(clo, -4.0f), cmax)) - 1, -2);
-        const int ch = min((int)floorf(fminf(fmaxf(chi, -4.0f), cmax)) + 2, dimC + 1);
+        cur = up ? hi + 1 : lo - 1;
         if (yh < yl || ch < cl) continue;  // block-uniform
         const int Ry = yh - yl + 1, Rc = ch - cl + 1;
         const bool staged = (long long)S * Ry * Rc <= kSlabCap;
@@ -703,18 +748,21 @@ __global__ __launch_bounds__(kSlabThreads) void forward_slab_kernel(ForwardJob J
                 const int ga = lo + a, gy = yl + y, gc = cl + c;
                 const bool in = (unsigned)ga < (unsigned)dimA && (unsigned)gy < (unsigned)J.Y &&
                                 (unsigned)gc < (unsigned)dimC;
+                // LDS DMA: the wave's lanes stage 64 consecutive box elements
+                // from e - lane on; a voxel outside the volume reads zero
+                // through an offset past the buffer's end (below 4 GiB) or
+                // from g_zero_word (64-bit addresses)
+                const int ebase = __builtin_amdgcn_readfirstlane(e - lane);
+                auto *dst = (__attribute__((address_space(3))) void *)(box + ebase);
                 if constexpr (kBuf) {
-                    // LDS DMA: the wave's lanes stage 64 consecutive box
-                    // elements from e - lane on; a voxel outside the volume
-                    // reads zero through an offset past the buffer's end
                     const unsigned voff =
                         in ? (unsigned)((ga * J.Y + gy) * dimC + gc) * 4u : 0xfffffffcu;
-                    const int ebase = __builtin_amdgcn_readfirstlane(e - lane);
-                    __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                        vrs, (__attribute__((address_space(3))) void *)(box + ebase), 4, voff, 0,
-                        0, 0);
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(vrs, dst, 4, voff, 0, 0, 0);
                 } else {
-                    box[e] = in ? src[((size_t)ga * J.Y + gy) * dimC + gc] : 0.0f;
+                    const float *gp =
+                        in ? src + (((size_t)ga * J.Y + gy) * dimC + gc) : g_zero_word;
+                    __builtin_amdgcn_global_load_lds(
+                        (const __attribute__((address_space(1))) void *)gp, dst, 4, 0, 0);
                 }
                 c += qc;
                 int carry = 0;
@@ -729,26 +777,34 @@ __global__ __launch_bounds__(kSlabThreads) void forward_slab_kernel(ForwardJob J
                     ++a;
                 }
             }
-            if constexpr (kBuf) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         }
         __syncthreads();
-        if (!along) continue;
-        if (ax_x)
-            slab_samples<true>(J, R, k, acc, box, staged, lo, hi, yl, cl, Ry, Rc);
-        else
-            slab_samples<false>(J, R, k, acc, box, staged, lo, hi, yl, cl, Ry, Rc);
-    }
-    for (; k < R.K; ++k) {
-        const float t = (float)k + 0.5f;
-        acc += trilinear0(J.vol, J.X, J.Y, J.Z, fmaf(t, R.s[0], R.b[0]),
-                          fmaf(t, R.s[1], R.b[1]), fmaf(t, R.s[2], R.b[2]));
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            if (!along[v]) continue;
+            if (ax_x)
+                slab_samples<true>(J, R[v], k[v], acc[v], box, staged, lo, hi, yl, cl, Ry, Rc);
+            else
+                slab_samples<false>(J, R[v], k[v], acc[v], box, staged, lo, hi, yl, cl, Ry, Rc);
+        }
     }
     if (!pix) return;
-    if (R.K > 0) acc *= (float)J.h;
-    const size_t idx = J.layout == CTR_LAYOUT_PROJECTIONS
-                           ? ((size_t)n * J.nv + iv) * J.nu + iu
-                           : ((size_t)iv * J.n_views + n) * J.nu + iu;
-    J.out[idx] = acc;
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        const int n = n0 + v;
+        if (n >= J.n_views) break;
+        for (; k[v] < R[v].K; ++k[v]) {
+            const float t = (float)k[v] + 0.5f;
+            acc[v] += trilinear0(J.vol, J.X, J.Y, J.Z, fmaf(t, R[v].s[0], R[v].b[0]),
+                                 fmaf(t, R[v].s[1], R[v].b[1]), fmaf(t, R[v].s[2], R[v].b[2]));
+        }
+        if (R[v].K > 0) acc[v] *= (float)J.h;
+        const size_t idx = J.layout == CTR_LAYOUT_PROJECTIONS
+                               ? ((size_t)n * J.nv + iv) * J.nu + iu
+                               : ((size_t)iv * J.n_views + n) * J.nu + iu;
+        J.out[idx] = acc[v];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1178,9 +1234,20 @@ int ctr_forward_project(void *stream, ctr_workspace *ws, int geometry, const flo
     // buffer resource or 64-bit addresses (CTR_OPTION_FORWARD_KERNEL =
     // CTR_KERNEL_RAY forces the ray kernel: diagnostics and tests)
     const bool slab = workspace_option(w, CTR_OPTION_FORWARD_KERNEL) != CTR_KERNEL_RAY;
+    // views per block: the shift r dtheta of the volume's corner voxels
+    // between consecutive views, r = half the volume's (x, z) diagonal in
+    // voxels.  Measured (ms; 1 / 2 / 4 views per block; the shift in
+    // brackets): 512^3 x 720 (3.2) 377 / 266 / 273, 256^3 x 360 (3.2) 24.6 /
+    // 17.6 / 18.0, 1024^3 x 1200 (3.8) 3088 / 2073 / 1980, 1440^3 x 720 (8.9)
+    // 5619 / 4191 / 4446, x 360 (17.8) 2812 / 2262 / 2764, x 96 (66) 755 /
+    // 1027 / 2099; four views (127 VGPRs) are not kept.
+    const double shift = 0.5 * sqrt((double)X * X + (double)Z * Z) * 2.0 * M_PI *
+                         fabs(rotation_count) / (double)n_views;
+    int views = shift <= 30.0 ? 2 : 1;
+    if (CTR_FWD_SLAB_VIEWS == 1 || CTR_FWD_SLAB_VIEWS == 2) views = CTR_FWD_SLAB_VIEWS;
     const long long slab_blocks =
         slab_blocks_per_view((nu + kSlabTU - 1) / kSlabTU, (nv + kSlabTV - 1) / kSlabTV) *
-        n_views;
+        ((n_views + views - 1) / views);
     CTR_REQUIRE(slab_blocks < (1LL << 31), CTR_ERR_INVALID_ARGUMENT,
                 "forward projection too large");
     const dim3 slab_grid((unsigned)slab_blocks);
@@ -1205,14 +1272,20 @@ int ctr_forward_project(void *stream, ctr_workspace *ws, int geometry, const flo
             volT = (const float *)t;
         }
     }
-    if (slab && buf)
-        forward_slab_kernel<true><<<slab_grid, kSlabThreads, 0, as_stream(stream)>>>(J, volT);
-    else if (slab)
-        forward_slab_kernel<false><<<slab_grid, kSlabThreads, 0, as_stream(stream)>>>(J, volT);
-    else if (buf)
+#define CTR_SLAB_LAUNCH(B, V)                                                              \
+    forward_slab_kernel<B, V><<<slab_grid, kSlabThreads, 0, as_stream(stream)>>>(J, volT)
+    if (slab && buf) {
+        if (views == 2) CTR_SLAB_LAUNCH(true, 2);
+        else CTR_SLAB_LAUNCH(true, 1);
+    } else if (slab) {
+        if (views == 2) CTR_SLAB_LAUNCH(false, 2);
+        else CTR_SLAB_LAUNCH(false, 1);
+    } else if (buf) {
         forward_kernel<true><<<grid, 256, 0, as_stream(stream)>>>(J);
-    else
+    } else {
         forward_kernel<false><<<grid, 256, 0, as_stream(stream)>>>(J);
+    }
+#undef CTR_SLAB_LAUNCH
     CTR_CHECK_LAUNCH();
     return CTR_OK;
 }

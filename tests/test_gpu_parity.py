@@ -260,6 +260,31 @@ def test_forward_slab_kernel_matches_ray_kernel(ctr, dev, step):
         assert torch.equal(slab_wide, ray), (g.kind, g.nu, step)
 
 
+@pytest.mark.parametrize("n_views", [6, 24, 240])
+def test_forward_slab_views_per_block_and_short_slabs(ctr, dev, n_views):
+    """The slab kernel stages one box for 1 or 2 consecutive views, chosen by
+    how far the volume's corner voxels move between views (37 voxels for 6
+    views of this volume: one view a block; 9.3 and 0.93 for 24 and 240
+    views: two), and shortens a slab whose
+    box would not fit LDS (detector pixels about three voxels wide at the
+    rotation axis here, so a 16 x 16 tile spans some 50 voxels).  Against the
+    ray kernel bit for bit, with the box filled through the buffer resource
+    and through 64-bit addresses."""
+    import torch
+    rng = np.random.default_rng(n_views)
+    X, Y, Z = 48, 40, 52
+    vol = _t(dev, rng.random((Z, Y, X)).astype(np.float32))
+    g = ctr.Geometry(ctr.CONE, nu=56, nv=48, n_views=n_views, X=X, Y=Y, Z=Z,
+                     detector_width=336.0, volume_width=48.0, source_distance=100.0,
+                     detector_distance=100.0, rotation_offset=0.15)
+    ray = ctr.forward_project(vol, g, step=1.0, ws=_forward_ws(ctr, True, False))
+    slab = ctr.forward_project(vol, g, step=1.0, ws=_forward_ws(ctr, False, False))
+    slab_wide = ctr.forward_project(vol, g, step=1.0, ws=_forward_ws(ctr, False, True))
+    assert float(ray.abs().max()) > 0.0
+    assert torch.equal(slab, ray), n_views
+    assert torch.equal(slab_wide, ray), n_views
+
+
 def test_fdk_end_to_end_ball(ctr, orc, dev):
     """Analytic projections of a uniform ball -> FDK recovers its density."""
     g = ctr.Geometry(ctr.CONE, nu=128, nv=128, n_views=180, X=64, Y=64, Z=64,
