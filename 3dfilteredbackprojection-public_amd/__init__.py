@@ -22,9 +22,7 @@ LIB_PATH = os.path.join(HERE, "libctrecon.so")
 
 CTR_OK = 0
 F32, F16, BF16 = 0, 1, 2
-INTERP_FAST, INTERP_FP32 = 0, 1
-(OPTION_INTERPOLATION, OPTION_BACKPROJECT_KERNEL, OPTION_FORWARD_KERNEL,
- OPTION_FORWARD_ADDRESSING) = 0, 1, 2, 3
+OPTION_BACKPROJECT_KERNEL, OPTION_FORWARD_KERNEL, OPTION_FORWARD_ADDRESSING = 0, 1, 2
 KERNEL_AUTO, KERNEL_LDS, KERNEL_GATHER, KERNEL_RAY, KERNEL_POINTER = 0, 1, 2, 3, 4
 RAM_LAK, SHEPP_LOGAN, COSINE = 0, 1, 2
 CONE, PARALLEL = 0, 1
@@ -190,11 +188,6 @@ class Workspace:
         rc = lib().ctr_workspace_get_option(self.handle, int(option))
         check(rc if rc < 0 else CTR_OK)
         return rc
-
-    def set_interpolation(self, mode):
-        """INTERP_FP32 or INTERP_FAST for the 16-bit staged forms of every
-        backprojection given this workspace; returns the previous mode."""
-        return self.set_option(OPTION_INTERPOLATION, mode)
 
     def close(self):
         if self.handle:

@@ -180,48 +180,33 @@ int build_geometry(hipStream_t s, ctr_workspace *ws, int geometry, int n_views, 
     return CTR_OK;
 }
 
-// Two horizontal taps of one staged row at byte offset `off`, as fp32.
-template <int DT>
-struct Taps;
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
-template <>
-struct Taps<CTR_F32> {
+// Two horizontal fp32 taps of one staged row at byte offset `off`.
+__device__ inline float2 f32_taps(const char *base, unsigned off)
+{
     struct __attribute__((aligned(4))) pair_t {
         float a, b;
     };
-    __device__ static inline float2 load(const char *base, unsigned off)
-    {
-        pair_t p = *reinterpret_cast<const pair_t *>(base + off);
-        return make_float2(p.a, p.b);
-    }
-};
-
-__device__ inline float2 f16_pair(unsigned w)
-{
-    return make_float2((float)__builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu)),
-                       (float)__builtin_bit_cast(_Float16, (unsigned short)(w >> 16)));
+    const pair_t p = *reinterpret_cast<const pair_t *>(base + off);
+    return make_float2(p.a, p.b);
 }
 
-__device__ inline float2 bf16_pair(unsigned w)
+// v + wv d of a 16-bit value-and-step texel (value v in the lower half, step
+// d to the next row in the upper half), in FP32.  fp16: both halves widened
+// inside one mixed-precision FMA (v_fma_mix_f32, exact widening).  bf16: the
+// word itself, read as FP32, is the step (its upper half was chosen for that
+// when the texel was written) and the value is the lower half shifted up.
+template <int DT>
+__device__ inline float vstep_lerp(unsigned w, float wv)
 {
-    return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
+    if constexpr (DT == CTR_BF16) {
+        return fmaf(wv, __uint_as_float(w), __uint_as_float(w << 16));
+    } else {
+        const half2_t h = __builtin_bit_cast(half2_t, w);
+        return fmaf(wv, (float)h.y, (float)h.x);
+    }
 }
-
-template <>
-struct Taps<CTR_F16> {
-    __device__ static inline float2 load(const char *base, unsigned off)
-    {
-        return f16_pair(*reinterpret_cast<const unsigned *>(base + off));
-    }
-};
-
-template <>
-struct Taps<CTR_BF16> {
-    __device__ static inline float2 load(const char *base, unsigned off)
-    {
-        return bf16_pair(*reinterpret_cast<const unsigned *>(base + off));
-    }
-};
 
 // Column quantities of one view, FP64 from the FP64 projection rows.
 struct ColumnView {
@@ -290,20 +275,36 @@ __device__ inline Run make_run(const ColumnView &c, double su_hi, double org_u, 
     return R;
 }
 
+// Direct taps of one column run: fp32 reads rows I and I + 1 (columns iu,
+// iu + 1) of the staged view at `top`; the 16-bit value-and-step texels need
+// row I only.
 template <int DT, int VY>
 __device__ inline void gather_run(float (&acc)[VY], const Run &R, const char *top,
-                                  const char *bot, unsigned pitch)
+                                  unsigned pitch)
 {
+    if constexpr (staged_is_16bit(DT)) {
 #pragma unroll
-    for (int t = 0; t < VY; ++t) {
-        const float x = __builtin_amdgcn_fmed3f(fmaf(R.dv, (float)t, R.f), R.lo, R.hi);
-        const float wv = __builtin_amdgcn_fractf(x);
-        const unsigned off = __umul24((unsigned)x, pitch) + R.base;
-        const float2 a = Taps<DT>::load(top, off);
-        const float2 b = Taps<DT>::load(bot, off);
-        const float left = fmaf(wv, b.x - a.x, a.x);
-        const float right = fmaf(wv, b.y - a.y, a.y);
-        acc[t] = fmaf(R.cR, right, fmaf(R.cL, left, acc[t]));
+        for (int t = 0; t < VY; ++t) {
+            const float x = __builtin_amdgcn_fmed3f(fmaf(R.dv, (float)t, R.f), R.lo, R.hi);
+            const float wv = __builtin_amdgcn_fractf(x);
+            const unsigned off = __umul24((unsigned)x, pitch) + R.base;
+            const unsigned *P = reinterpret_cast<const unsigned *>(top + off);
+            const float left = vstep_lerp<DT>(P[0], wv);
+            const float right = vstep_lerp<DT>(P[1], wv);
+            acc[t] = fmaf(R.cR, right, fmaf(R.cL, left, acc[t]));
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < VY; ++t) {
+            const float x = __builtin_amdgcn_fmed3f(fmaf(R.dv, (float)t, R.f), R.lo, R.hi);
+            const float wv = __builtin_amdgcn_fractf(x);
+            const unsigned off = __umul24((unsigned)x, pitch) + R.base;
+            const float2 a = f32_taps(top, off);
+            const float2 b = f32_taps(top + pitch, off);
+            const float left = fmaf(wv, b.x - a.x, a.x);
+            const float right = fmaf(wv, b.y - a.y, a.y);
+            acc[t] = fmaf(R.cR, right, fmaf(R.cL, left, acc[t]));
+        }
     }
 }
 
@@ -356,7 +357,7 @@ struct BpBatch {
 // Direct-tap kernel: taps straight from the staged buffer (global loads
 // through L1/L2), coordinates clamped to the staged view: per voxel the v
 // FMA, a clamp, fract, cvt, mad24, two loads, two lerps and two FMAs (11
-// VALU for fp32).  Used for short runs (Y < 32, the 2D fan / parallel
+// VALU for fp32, 9 for fp16).  Used for short runs (Y < 32, the 2D fan / parallel
 // workloads) and as the reference variant of the LDS kernel.
 // ---------------------------------------------------------------------------
 template <int DT, int VY>
@@ -384,7 +385,7 @@ __global__ __launch_bounds__(256) void backproject_kernel(BpJob J)
             column_view64(J.geom64 + (size_t)kGeom64 * n, i, k, j0, vs);
         const Run R = make_run(c, su_hi, 0.0, 0, J.nv + 1, 0, pitch4, su_hi);
         const char *top = vbase + (size_t)n * view_bytes;
-        gather_run<DT, VY>(acc, R, top, top + pitch4, pitch4);
+        gather_run<DT, VY>(acc, R, top, pitch4);
     }
     store_run<VY>(acc, J.vol, i, j0, k, J.X, J.Y, J.Z, J.accumulate, J.clip);
 }
@@ -411,26 +412,19 @@ __global__ __launch_bounds__(256) void backproject_kernel(BpJob J)
 // are loaded from the (zero) border of the staged view by clamping the
 // source row and column, so the per-voxel row coordinate needs no clamp.
 //
-// Per voxel:
-//   fp32 -- v FMA, fract, cvt, mad24, the taps (two ds_read2_b32), two
-//     lerps and two FMAs into the accumulator (10 VALU);
-//   bf16 -- v FMA, fract, cvt, mad24, one ds_read2_b32 of the (u, u + 1)
-//     pairs of rows I and I + 1, three v_dot2_f32_bf16 with the column's
-//     bf16 (1 - wu, wu) weights (top row onto the accumulator, and the
-//     bottom-minus-top difference) and one FMA with the row weight
-//     (9 VALU; bf16 weights add about 2^-9 relative error per sample);
-//   the fp32 and fp16 loops are software-pipelined: the LDS reads of the
-//   next batch of voxels are issued before the current batch is combined;
-//   fp16 -- v FMA, fract, cvt, mad24, one ds_read2_b32 of the (u, u + 1)
-//     pairs of rows I and I + 1, the weight to f16 (one v_cvt_pk_f16_f32 per
-//     two voxels), a packed f16 difference and lerp, and v_dot2_f32_f16 of
-//     the lerped pair with the column's f16 (1 - wu, wu) weights into the
-//     FP32 accumulator (7.5 VALU).  Those f16
-//     weights hold the distance weight times view_scale / max(view_scale)
-//     (at most (SOD / (SOD - r))^2, r the volume's half diagonal; the host
-//     keeps this path for geometries where that stays far below the f16
-//     range), and the run is rescaled by scale * max(view_scale) at the
-//     store.
+// Per voxel, all arithmetic FP32:
+//   fp32 -- v FMA, fract, cvt, mad24, the taps (two ds_read2_b32: rows I and
+//     I + 1, columns u and u + 1), two subtractions and two lerps for the
+//     columns, two FMAs into the accumulator with the column's (1 - wu, wu)
+//     weights (10 VALU, six of them FMA-class);
+//   fp16 -- v FMA, fract, cvt, mad24, one ds_read2_b32 of the value-and-step
+//     texels of columns u and u + 1 in row I, each column's vertical lerp
+//     v + wv d as one v_fma_mix_f32 straight from the texel's f16 halves,
+//     the two FMAs into the accumulator (8 VALU, four FMA-class);
+//   bf16 -- as fp16, with one shift per texel to widen the value (the step
+//     is the texel read as FP32): 10 VALU, four FMA-class.
+// The LDS reads are software-pipelined: those of the next batch of voxels
+// are issued before the current batch is combined.
 // ---------------------------------------------------------------------------
 // LDS geometry per staged format: staged columns, row pitch in dwords and
 // rows per buffer.  A 16 x 16 column tile spans at most 44 detector columns
@@ -444,15 +438,16 @@ __global__ __launch_bounds__(256) void backproject_kernel(BpJob J)
 // (1, -1) neighbour, s = 5 (69) for |du| >= 5 pairs two rows apart, s = 11
 // (75) and s = -11 (53) only for |du| >= 10 or three rows apart.  Two
 // buffers of `rows` rows bound the workgroups per CU (160 KB of LDS): fp32
-// at 53 x 84 fits four (its 120 VGPRs allow no more), 16-bit pairs at
+// at 53 x 84 fits four (its 120 VGPRs allow no more), 16-bit texels at
 // 51 x 72 five.  Measured (GUPS, cols/pitch/rows; 512^3 x 720, 1024^3 x 1200,
-// 1440^3 x 1440):
+// 1440^3 x 1440; the 16-bit rows with the earlier (q[u], q[u + 1]) pair
+// texels, two rows read per voxel, and 16-bit arithmetic):
 //   fp32  64/65/96 1712 (512^3 only), 64/75/84 2825 3399 3480,
 //         48/53/84 2909 3637 3795, 48/51/84 2805 3625 3804
 //   fp16  64/65/96 2664 (512^3 only), 64/69/70 3618 4149 4009,
 //         48/53/70 3738 4296 4259, 48/51/72 3760 4302 4351
 // fp32 reads four taps per voxel and is the most sensitive to conflicts;
-// 16-bit pairs read two and profit more from the fifth workgroup.  Every
+// 16-bit texels read two and profit more from the fifth workgroup.  Every
 // 32-voxel footprint of the 512^3 scan fits 72 rows bar 0.01% (98.9% of
 // the 1440^3 scan); larger ones take the direct-tap pass.
 #ifndef CTR_LDS_COLS32
@@ -482,32 +477,18 @@ __global__ __launch_bounds__(256) void backproject_kernel(BpJob J)
 #ifndef CTR_LDS_TY
 #define CTR_LDS_TY 32
 #endif
-// Voxels per software-pipelined batch of LDS reads (fp32 / packed fp16).
-// Measured on 512^3 x 720 and 1024^3 x 1200 (GUPS): fp32 batch 4 2767 /
-// 3254, batch 2 2792 / 3389, batch 8 2625 / 3071 (later, on the current
-// kernel: batch 1 2937 / 3635 against batch 2 2943 / 3641); packed fp16
-// without pipelining 3343 / 3751, batch 2 3556 / 4146, batch 4 3487 / 4120.
+// Voxels per software-pipelined batch of LDS reads.  Measured on 512^3 x
+// 720 and 1024^3 x 1200 (GUPS): fp32 batch 4 2767 / 3254, batch 2 2792 /
+// 3389, batch 8 2625 / 3071 (later, on the current kernel: batch 1 2937 /
+// 3635 against batch 2 2943 / 3641).
 #ifndef CTR_F32_BATCH
 #define CTR_F32_BATCH 2
 #endif
-#ifndef CTR_F16_BATCH
-#define CTR_F16_BATCH 2
-#endif
-// The same for the FP32-interpolation branch of the 16-bit pairs.  Measured
-// (GUPS, 512^3 / 1024^3 / 1440^3): fp16 batch 1 3420 / 3836 / 3965, batch 2
-// 3379 / 3682 / 3879, batch 4 3381 / 3727 at 512^3 / 1024^3; bf16 batch 2
-// 3203 / 3578 / 3791, batch 1 3172 / 3568 / 3790.
-#ifndef CTR_WF16_BATCH
-#define CTR_WF16_BATCH 1  // fp16 pairs, FP32 interpolation
-#endif
-#ifndef CTR_WBF16_BATCH
-#define CTR_WBF16_BATCH 2  // bf16 pairs, FP32 interpolation
+// The same for the 16-bit value-and-step texels.
+#ifndef CTR_16_BATCH
+#define CTR_16_BATCH 1
 #endif
 constexpr int kRectChunk = 128; // views per refill of the rectangle table
-// Tile variants with 16-bit interpolation arithmetic (the plain format tags
-// CTR_F16 / CTR_BF16 widen the pairs and interpolate in FP32):
-constexpr int kF16Packed = 16;  // fp16 pairs, packed f16 lerp, v_dot2_f32_f16 column weights
-constexpr int kBF16Dot = 17;    // bf16 pairs, v_dot2_f32_bf16 with bf16 column weights
 
 template <int DT>
 struct LdsConfig {
@@ -537,35 +518,24 @@ struct LdsLayout {
     static_assert(kCols <= 64 && kPitch >= kCols + 1 && kPitch + 1 <= 255, "ds_read2 offsets");
 };
 
-// Taps at LDS byte address `addr`: rows I (top) and I + 1 (bot), columns
-// (u, u + 1) as fp32.
-template <int DT>
-__device__ inline void lds_taps(const unsigned *L, unsigned addr, float2 &top, float2 &bot)
+// fp32 taps at LDS byte address `addr`: rows I (top) and I + 1 (bot),
+// columns (u, u + 1).
+__device__ inline void lds_taps_f32(const unsigned *L, unsigned addr, float2 &top, float2 &bot)
 {
-    constexpr int P = LdsConfig<DT>::kPitch;
-    if constexpr (DT == CTR_F32) {
-        const float *F = reinterpret_cast<const float *>(reinterpret_cast<const char *>(L) + addr);
-        top = make_float2(F[0], F[1]);
-        bot = make_float2(F[P], F[P + 1]);
-    } else {
-        const unsigned *Q =
-            reinterpret_cast<const unsigned *>(reinterpret_cast<const char *>(L) + addr);
-        if constexpr (DT == CTR_BF16) {
-            top = bf16_pair(Q[0]);
-            bot = bf16_pair(Q[P]);
-        } else {
-            top = f16_pair(Q[0]);
-            bot = f16_pair(Q[P]);
-        }
-    }
+    constexpr int P = LdsConfig<CTR_F32>::kPitch;
+    const float *F = reinterpret_cast<const float *>(reinterpret_cast<const char *>(L) + addr);
+    top = make_float2(F[0], F[1]);
+    bot = make_float2(F[P], F[P + 1]);
 }
 
 // Detector rectangle of one tile for view n: (u0, v0, rows, mode), staged
 // coordinates.  Bounds are clamped to [-1024, n + 1024] first so the int
 // conversions stay exact; a clamped rectangle is still classified correctly
-// because the clamp margin exceeds both buffer limits.
+// because the clamp margin exceeds both buffer limits.  `below` is 1 when
+// the format's taps include row I + 1 (fp32), 0 when a texel of row I
+// carries both vertical taps (16-bit value-and-step texels).
 __device__ inline int4 tile_rect(const BpJob &J, const float4 *foot, size_t foot_stride, int n,
-                                 float ja, float jb, float cv1, int hmax, int cmax)
+                                 float ja, float jb, float cv1, int hmax, int cmax, int below)
 {
     const float4 fp = foot[(size_t)n * foot_stride];
     const float vj = J.geom[3 * n].w;
@@ -576,9 +546,12 @@ __device__ inline int4 tile_rect(const BpJob &J, const float4 *foot, size_t foot
     const int u0 = (int)floorf(__builtin_amdgcn_fmed3f(fp.x - 0.05f, -1024.0f, ul));
     const int u1 = (int)floorf(__builtin_amdgcn_fmed3f(fp.y + 0.05f, -1024.0f, ul)) + 1;
     const int v0 = (int)floorf(__builtin_amdgcn_fmed3f(cv1 + lo - 0.05f, -1024.0f, vl));
-    const int v1 = (int)floorf(__builtin_amdgcn_fmed3f(cv1 + hi + 0.05f, -1024.0f, vl)) + 1;
+    const int v1 = (int)floorf(__builtin_amdgcn_fmed3f(cv1 + hi + 0.05f, -1024.0f, vl)) + below;
+    // skip: every tap in the zero border.  Staged row 0 is a zero row of
+    // fp32, but a 16-bit texel there holds (0, first data row), so for the
+    // 16-bit formats only rows < 0 are zero above the data.
     int mode;
-    if (u1 <= 0 || v1 <= 0 || u0 >= J.nu + 1 || v0 >= J.nv + 1)
+    if (u1 <= 0 || v1 < below || u0 >= J.nu + 1 || v0 >= J.nv + 1)
         mode = RECT_SKIP;
     else if (u1 - u0 < cmax && v1 - v0 < hmax)
         mode = RECT_LDS;
@@ -606,9 +579,10 @@ __device__ inline int4 read_rect(const int4 *tab, int c)
 
 // Rows v0 .. v0 + h - 1 of one staged view into an LDS row buffer, row r by
 // wave r mod 4, `cols` columns from u0 (one per lane).  Rows outside the
-// view are read from its first or last row, columns outside it from its
-// last column: all zero in every texel format (column 0 is not -- a 16-bit
-// pair texel there holds (0, q[1])).  The loads go through a buffer
+// view are read from its last row, columns outside it from its last
+// column: both zero in every texel format (row 0 is not -- a 16-bit
+// value-and-step texel there holds (0, first data row)).  The loads go
+// through a buffer
 // resource spanning exactly the view (raw 32-bit format), with the row
 // offset in an SGPR and the lane's column in the VGPR offset: no VALU work
 // per row, and any address outside the view would read zero.
@@ -625,7 +599,7 @@ __device__ inline void stage_rect(const char *view, unsigned view_bytes, int pit
     const unsigned col4 = (unsigned)(col < 0 || col >= pitch_u ? pitch_u - 1 : col) * 4u;
     const unsigned row_bytes = (unsigned)pitch_u * 4u;
     for (int r = wave; r < h; r += 4) {
-        const int srow = min(max(v0 + r, 0), pitch_v - 1);
+        const int srow = v0 + r < 0 ? pitch_v - 1 : min(v0 + r, pitch_v - 1);
         __builtin_amdgcn_raw_ptr_buffer_load_lds(
             rsrc, (__attribute__((address_space(3))) void *)(buf + r * PITCH), 4, col4,
             (unsigned)srow * row_bytes, 0, 0);
@@ -653,65 +627,15 @@ __device__ inline LdsRun lds_run(const ColumnView &c, int u0, int v0, unsigned b
     return R;
 }
 
-typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
-typedef __fp16 fp16x2_t __attribute__((ext_vector_type(2)));
-typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 
 template <int DT, int TY>
 __device__ inline void lds_run_accumulate(float (&acc)[TY], const LdsRun &R, const unsigned *lds,
                                           unsigned lp4)
 {
-    if constexpr (DT == kF16Packed) {
-        const half2_t W = {(_Float16)R.cL, (_Float16)R.cR};
-        // software-pipelined like the fp32 branch: the pair reads of batch
-        // b + 1 are in flight while batch b is interpolated; the row weights
-        // of two voxels share one packed conversion
-        constexpr int kB = CTR_F16_BATCH;
-        static_assert(kB % 2 == 0 && TY % kB == 0, "batch");
-        unsigned ta[2][kB], tb[2][kB];
-        float tw[2][kB];
-        auto issue = [&](int set, int t0) {
-#pragma unroll
-            for (int q = 0; q < kB; ++q) {
-                const float x = fmaf(R.dv, (float)(t0 + q), R.f);
-                tw[set][q] = __builtin_amdgcn_fractf(x);
-                const unsigned *P = reinterpret_cast<const unsigned *>(
-                    reinterpret_cast<const char *>(lds) + __umul24((unsigned)x, lp4) + R.base);
-                ta[set][q] = P[0];
-                tb[set][q] = P[LdsConfig<DT>::kPitch];
-            }
-        };
-        issue(0, 0);
-#pragma unroll
-        for (int bt = 0; bt < TY / kB; ++bt) {
-            if (bt + 1 < TY / kB) issue((bt + 1) & 1, (bt + 1) * kB);
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int q = 0; q < kB; q += 2) {
-                // one v_cvt_pk_f16_f32 for both weights: the empty asm keeps
-                // the compiler from splitting it into two scalar conversions
-                unsigned wbits = __builtin_bit_cast(
-                    unsigned, half2_t{(_Float16)tw[bt & 1][q], (_Float16)tw[bt & 1][q + 1]});
-                asm volatile("" : "+v"(wbits));
-                const half2_t wp = __builtin_bit_cast(half2_t, wbits);
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const half2_t a = __builtin_bit_cast(half2_t, ta[bt & 1][q + h]);
-                    const half2_t b = __builtin_bit_cast(half2_t, tb[bt & 1][q + h]);
-                    const half2_t w = h ? half2_t{wp.y, wp.y} : half2_t{wp.x, wp.x};
-                    const half2_t lr = __builtin_elementwise_fma(w, b - a, a);
-                    acc[bt * kB + q + h] =
-                        __builtin_amdgcn_fdot2(__builtin_bit_cast(fp16x2_t, lr),
-                                               __builtin_bit_cast(fp16x2_t, W),
-                                               acc[bt * kB + q + h], false);
-                }
-            }
-            __builtin_amdgcn_sched_barrier(0);
-        }
-    } else if constexpr (DT == CTR_F32) {
-        // fp32 reads four taps per voxel: software-pipelined in batches of
-        // kB voxels (taps of batch b + 1 in flight while batch b is
-        // interpolated), the scheduling barriers keep that order
+    // Software-pipelined in batches of kB voxels: the taps of batch b + 1
+    // are in flight while batch b is interpolated; the scheduling barriers
+    // keep that order.
+    if constexpr (DT == CTR_F32) {
         constexpr int kB = CTR_F32_BATCH;
         float2 ta[2][kB], tb[2][kB];
         float tw[2][kB];
@@ -720,7 +644,7 @@ __device__ inline void lds_run_accumulate(float (&acc)[TY], const LdsRun &R, con
             for (int q = 0; q < kB; ++q) {
                 const float x = fmaf(R.dv, (float)(t0 + q), R.f);
                 tw[set][q] = __builtin_amdgcn_fractf(x);
-                lds_taps<DT>(lds, __umul24((unsigned)x, lp4) + R.base, ta[set][q], tb[set][q]);
+                lds_taps_f32(lds, __umul24((unsigned)x, lp4) + R.base, ta[set][q], tb[set][q]);
             }
         };
         issue(0, 0);
@@ -738,55 +662,13 @@ __device__ inline void lds_run_accumulate(float (&acc)[TY], const LdsRun &R, con
             }
             __builtin_amdgcn_sched_barrier(0);
         }
-    } else if constexpr (DT == kBF16Dot) {
-        // bf16 pairs: each row's (u, u + 1) pair is weighted by a
-        // v_dot2_f32_bf16 with the column's bf16 (1 - wu, wu) w2 weights:
-        // acc += W.a in place, then d = W.b - W.a and acc += wv d.  No
-        // widening of the pairs is needed.
-        const bf16x2_t W = {(__bf16)R.cL, (__bf16)R.cR};
-        const bf16x2_t Wn = {-W.x, -W.y};
-        constexpr int kB = CTR_F32_BATCH;
-        unsigned ta[2][kB], tb[2][kB];
-        float tw[2][kB];
-        auto issue = [&](int set, int t0) {
-#pragma unroll
-            for (int q = 0; q < kB; ++q) {
-                const float x = fmaf(R.dv, (float)(t0 + q), R.f);
-                tw[set][q] = __builtin_amdgcn_fractf(x);
-                const unsigned *P = reinterpret_cast<const unsigned *>(
-                    reinterpret_cast<const char *>(lds) + __umul24((unsigned)x, lp4) + R.base);
-                ta[set][q] = P[0];
-                tb[set][q] = P[LdsConfig<DT>::kPitch];
-            }
-        };
-        issue(0, 0);
-#pragma unroll
-        for (int bt = 0; bt < TY / kB; ++bt) {
-            if (bt + 1 < TY / kB) issue((bt + 1) & 1, (bt + 1) * kB);
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int q = 0; q < kB; ++q) {
-                const bf16x2_t a = __builtin_bit_cast(bf16x2_t, ta[bt & 1][q]);
-                const bf16x2_t b = __builtin_bit_cast(bf16x2_t, tb[bt & 1][q]);
-                float &A = acc[bt * kB + q];
-                A = __builtin_amdgcn_fdot2_f32_bf16(a, W, A, false);
-                const float d = __builtin_amdgcn_fdot2_f32_bf16(
-                    a, Wn, __builtin_amdgcn_fdot2_f32_bf16(b, W, 0.0f, false), false);
-                A = fmaf(tw[bt & 1][q], d, A);
-            }
-            __builtin_amdgcn_sched_barrier(0);
-        }
     } else {
-        // fp16 / bf16 pairs widened to FP32 and interpolated like the fp32
-        // branch; the pairs are read as raw dwords and widened after the
-        // scheduling barrier, so the reads of batch b + 1 stay in flight
-        static_assert(DT == CTR_F16 || DT == CTR_BF16, "tile variant");
-        constexpr int kB = DT == CTR_F16 ? CTR_WF16_BATCH : CTR_WBF16_BATCH;
-        // 1.0 held in an SGPR: b * one - a on widened f16 halves then stays
-        // one mixed-precision FMA (a literal 1.0 is folded into a
-        // subtraction of two converted values)
-        float one = 1.0f;
-        asm("" : "+s"(one));
+        // 16-bit value-and-step texels: columns u and u + 1 of row I (one
+        // ds_read2_b32 of neighbouring dwords); each column's vertical lerp
+        // v + wv d is one FMA on the texel, then the two weighted
+        // accumulations
+        static_assert(DT == CTR_F16 || DT == CTR_BF16, "staged format");
+        constexpr int kB = CTR_16_BATCH;
         unsigned ta[2][kB], tb[2][kB];
         float tw[2][kB];
         auto issue = [&](int set, int t0) {
@@ -797,7 +679,7 @@ __device__ inline void lds_run_accumulate(float (&acc)[TY], const LdsRun &R, con
                 const unsigned *P = reinterpret_cast<const unsigned *>(
                     reinterpret_cast<const char *>(lds) + __umul24((unsigned)x, lp4) + R.base);
                 ta[set][q] = P[0];
-                tb[set][q] = P[LdsConfig<DT>::kPitch];
+                tb[set][q] = P[1];
             }
         };
         issue(0, 0);
@@ -807,25 +689,11 @@ __device__ inline void lds_run_accumulate(float (&acc)[TY], const LdsRun &R, con
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int q = 0; q < kB; ++q) {
-                const unsigned ra = ta[bt & 1][q], rb = tb[bt & 1][q];
                 const float wv = tw[bt & 1][q];
+                const float left = vstep_lerp<DT>(ta[bt & 1][q], wv);
+                const float right = vstep_lerp<DT>(tb[bt & 1][q], wv);
                 float &A = acc[bt * kB + q];
-                if constexpr (DT == CTR_F16) {
-                    // b - a and the lerp straight from the f16 halves: the
-                    // mixed-precision FMA widens them exactly, so this is
-                    // the FP32 arithmetic of the fp32 branch without the
-                    // four conversions
-                    const half2_t a = __builtin_bit_cast(half2_t, ra);
-                    const half2_t b = __builtin_bit_cast(half2_t, rb);
-                    const float left = fmaf(wv, fmaf((float)b.x, one, -(float)a.x), (float)a.x);
-                    const float right = fmaf(wv, fmaf((float)b.y, one, -(float)a.y), (float)a.y);
-                    A = fmaf(R.cR, right, fmaf(R.cL, left, A));
-                } else {
-                    const float2 a = bf16_pair(ra), b = bf16_pair(rb);
-                    const float left = fmaf(wv, b.x - a.x, a.x);
-                    const float right = fmaf(wv, b.y - a.y, a.y);
-                    A = fmaf(R.cR, right, fmaf(R.cL, left, A));
-                }
+                A = fmaf(R.cR, right, fmaf(R.cL, left, A));
             }
             __builtin_amdgcn_sched_barrier(0);
         }
@@ -846,8 +714,7 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
 {
     using L = LdsLayout<DT>;
     constexpr int HMAX = L::kRows, CMAX = L::kCols;
-    // staged texel format of the variant
-    constexpr int kTapDT = DT == kF16Packed ? CTR_F16 : DT == kBF16Dot ? CTR_BF16 : DT;
+    constexpr int kBelow = DT == CTR_F32 ? 1 : 0;  // taps of row I + 1 read from LDS
     int4 *rects = reinterpret_cast<int4 *>(lds);
     unsigned *bufs = lds + L::kRectBytes / 4;
     const int pitch_u = J.pitch_u, pitch_v = J.pitch_v, N = J.n_views;
@@ -867,21 +734,6 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
     const unsigned view_bytes32 = (unsigned)view_bytes;  // < 2^31, checked by the host
     const unsigned lp4 = J.lds_pitch4;
 
-    // accumulation scale: fp16 weights are relative to the largest view scale
-    float wscale = J.scale, out_scale = 1.0f;
-    if constexpr (DT == kF16Packed) {
-        float m = 0.0f;
-        for (int t = threadIdx.x; t < N; t += 256) m = fmaxf(m, fabsf(J.view_scale[t]));
-        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-        float *part = reinterpret_cast<float *>(lds);
-        if (lane == 0) part[wave] = m;
-        __syncthreads();
-        m = fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
-        __syncthreads();
-        wscale = m > 0.0f ? 1.0f / m : 0.0f;
-        out_scale = J.scale * m;
-    }
-
     // views are visited from v0 on, wrapping (v0 = 0: in order)
     const int v0 = first_view(N);
     auto vw = [&](int pos) {
@@ -894,7 +746,7 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
     for (int t = threadIdx.x; t < L::kBufDwords / 2; t += 256)
         reinterpret_cast<uint4 *>(bufs)[t] = make_uint4(0u, 0u, 0u, 0u);
     if (threadIdx.x <= kRectChunk && (int)threadIdx.x < N)
-        rects[threadIdx.x] = tile_rect(J, foot, foot_stride, vw(threadIdx.x), ja, jb, cv1, HMAX, CMAX);
+        rects[threadIdx.x] = tile_rect(J, foot, foot_stride, vw(threadIdx.x), ja, jb, cv1, HMAX, CMAX, kBelow);
     __syncthreads();
 
     float acc[TY];
@@ -923,7 +775,7 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
         }
         if (rect_mode(cur) == RECT_LDS) {
             const int view = vw(n);
-            float vs = wscale;
+            float vs = J.scale;
             if (J.view_scale) vs *= as_constant(J.view_scale)[view];
             const ColumnView cv = column_view64(J.geom64 + (size_t)kGeom64 * view, i, k, j0, vs);
             const unsigned buf_byte = (unsigned)(L::kRectBytes + (n & 1) * L::kBufDwords * 4);
@@ -937,7 +789,7 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
             n0 += kRectChunk;
             if (threadIdx.x <= kRectChunk && n0 + (int)threadIdx.x < N)
                 rects[threadIdx.x] =
-                    tile_rect(J, foot, foot_stride, vw(n0 + threadIdx.x), ja, jb, cv1, HMAX, CMAX);
+                    tile_rect(J, foot, foot_stride, vw(n0 + threadIdx.x), ja, jb, cv1, HMAX, CMAX, kBelow);
             __syncthreads();
         }
         cur = nxt;
@@ -950,19 +802,15 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
 #pragma unroll 1
         for (int n = 0; n < N; ++n) {
             const int mode = __builtin_amdgcn_readfirstlane(
-                rect_mode(tile_rect(J, foot, foot_stride, n, ja, jb, cv1, HMAX, CMAX)));
+                rect_mode(tile_rect(J, foot, foot_stride, n, ja, jb, cv1, HMAX, CMAX, kBelow)));
             if (mode != RECT_DIRECT) continue;
-            float vs = wscale;
+            float vs = J.scale;
             if (J.view_scale) vs *= J.view_scale[n];
             const ColumnView cv = column_view64(J.geom64 + (size_t)kGeom64 * n, i, k, j0, vs);
             const Run R = make_run(cv, su_hi, 0.0, 0, J.nv + 1, 0, pitch4, su_hi);
             const char *top = vbase + (size_t)n * view_bytes;
-            gather_run<kTapDT, TY>(acc, R, top, top + pitch4, pitch4);
+            gather_run<DT, TY>(acc, R, top, pitch4);
         }
-    }
-    if constexpr (DT == kF16Packed) {
-#pragma unroll
-        for (int t = 0; t < TY; ++t) acc[t] *= out_scale;
     }
     store_run<TY>(acc, J.vol, i, j0, k, J.X, J.Y, J.Z, J.accumulate, J.clip);
 }
@@ -1086,19 +934,6 @@ static size_t footprint_bytes(int X, int Z, int n_views)
     return sizeof(float4) * (size_t)((X + 15) / 16) * ((Z + 15) / 16) * n_views;
 }
 
-// The packed-f16 LDS variant stores the column's distance weight (times a
-// view scale ratio <= 1) in f16: usable while (SOD / (SOD - r))^2, r the
-// volume's half diagonal, stays far below the f16 range.
-static bool f16_weights_fit(const BpJob &J, int geometry, double sod, double volume_width)
-{
-    if (geometry != CTR_GEOMETRY_CONE) return true;
-    const double d = volume_width / J.X;
-    const double rx = 0.5 * J.X * d, rz = 0.5 * J.Z * d;
-    const double r = sqrt(rx * rx + rz * rz);
-    const double g = sod / (sod - r);
-    return g * g < 1024.0;
-}
-
 template <int DT>
 static void launch_lds(hipStream_t s, BpJob J, unsigned blocks)
 {
@@ -1113,29 +948,13 @@ static void launch_lds_packed(hipStream_t s, BpBatch B, unsigned blocks)
     backproject_lds_packed_kernel<DT, kTY><<<blocks, 256, 0, s>>>(B);
 }
 
-// LDS tile variant of a staged format under the workspace's interpolation
-// arithmetic: the plain format (FP32 interpolation) or the kF16Packed /
-// kBF16Dot variants (CTR_INTERP_FAST).
-static int lds_variant(int interpolation, int staged_dtype, bool f16_weights_ok)
-{
-    if (interpolation == CTR_INTERP_FP32) return staged_dtype;
-    if (staged_dtype == CTR_F16) return f16_weights_ok ? kF16Packed : CTR_F16;
-    if (staged_dtype == CTR_BF16) return kBF16Dot;
-    return staged_dtype;
-}
-
 template <int DT>
-static int launch_bp(hipStream_t s, const BpJob &J, bool lds, int variant)
+static int launch_bp(hipStream_t s, const BpJob &J, bool lds)
 {
     if (lds) {
         const long long blocks = (long long)J.gx * J.gy * ((J.Y + kTY - 1) / kTY);
         CTR_REQUIRE(blocks < (1LL << 31), CTR_ERR_INVALID_ARGUMENT, "volume too large");
-        if (variant == kF16Packed)
-            launch_lds<kF16Packed>(s, J, (unsigned)blocks);
-        else if (variant == kBF16Dot)
-            launch_lds<kBF16Dot>(s, J, (unsigned)blocks);
-        else
-            launch_lds<DT>(s, J, (unsigned)blocks);
+        launch_lds<DT>(s, J, (unsigned)blocks);
         CTR_CHECK_LAUNCH();
         return CTR_OK;
     }
@@ -1166,8 +985,8 @@ static int check_backproject(int geometry, const void *staged, int staged_dtype,
 {
     CTR_REQUIRE(staged && volume, CTR_ERR_NULL_POINTER,
                 "[CTAnalyzer Error] ctr_backproject: projections or volume is NULL");
-    CTR_REQUIRE(staged_dtype == CTR_F32 || staged_dtype == CTR_F16 || staged_dtype == CTR_BF16,
-                CTR_ERR_INVALID_ARGUMENT, "unknown staged dtype %d", staged_dtype);
+    CTR_REQUIRE(staged_dtype_valid(staged_dtype), CTR_ERR_INVALID_ARGUMENT,
+                "unknown staged dtype %d", staged_dtype);
     CTR_REQUIRE(staged_dtype == CTR_F32 || view_scale, CTR_ERR_NULL_POINTER,
                 "view_scale is required for 16-bit staged projections");
     int rc = check_geometry(geometry, sod, odd, detector_width, volume_width, X, Y, Z, nu, nv,
@@ -1184,7 +1003,7 @@ static int check_backproject(int geometry, const void *staged, int staged_dtype,
 // at `geom_buf` (kGeomBytesPerView per view) and, when the LDS kernel runs,
 // the tile footprint table at `foot_buf` (footprint_bytes).  Both tables are
 // written whole before they are read, so no scratch needs clearing.
-static int backproject_with(hipStream_t s, int kernel_option, int interpolation, int geometry,
+static int backproject_with(hipStream_t s, int kernel_option, int geometry,
                             const void *staged, int staged_dtype, const float *view_scale,
                             int nu, int nv, int n_views, double sod, double odd,
                             double detector_width, double volume_width, double rotation_count,
@@ -1201,12 +1020,10 @@ static int backproject_with(hipStream_t s, int kernel_option, int interpolation,
                        accumulate, clip_negative);
     const bool lds = use_lds_kernel(kernel_option, Y);
     if (lds && (rc = build_footprints(s, J, (float4 *)foot_buf))) return rc;
-    const int variant =
-        lds_variant(interpolation, staged_dtype, f16_weights_fit(J, geometry, sod, volume_width));
     switch (staged_dtype) {
-    case CTR_F32: return launch_bp<CTR_F32>(s, J, lds, variant);
-    case CTR_F16: return launch_bp<CTR_F16>(s, J, lds, variant);
-    default: return launch_bp<CTR_BF16>(s, J, lds, variant);
+    case CTR_F32: return launch_bp<CTR_F32>(s, J, lds);
+    case CTR_F16: return launch_bp<CTR_F16>(s, J, lds);
+    default: return launch_bp<CTR_BF16>(s, J, lds);
     }
 }
 
@@ -1266,8 +1083,7 @@ int ctr_backproject(void *stream, ctr_workspace *ws, int geometry, const void *s
     if (use_lds_kernel(kernel_option, Y) &&
         (rc = workspace_get(w, SLOT_FOOTPRINT, footprint_bytes(X, Z, n_views), &foot)))
         return rc;
-    return backproject_with(as_stream(stream), kernel_option,
-                            workspace_option(w, CTR_OPTION_INTERPOLATION), geometry, staged,
+    return backproject_with(as_stream(stream), kernel_option, geometry, staged,
                             staged_dtype, view_scale, nu, nv, n_views, source_distance,
                             detector_distance, detector_width, volume_width, rotation_count,
                             rotation_offset, volume, X, Y, Z, scale, accumulate, clip_negative,
@@ -1295,8 +1111,7 @@ int ctr_backproject_noalloc(void *stream, int geometry, const void *staged, int 
                 (X + 15) / 16, (Z + 15) / 16, geom_bytes + foot, kNoAllocScratchBytes);
     void *base = nullptr;
     if ((rc = scratch_address(&base))) return rc;
-    return backproject_with(as_stream(stream), kernel_option,
-                            default_option(CTR_OPTION_INTERPOLATION), geometry, staged,
+    return backproject_with(as_stream(stream), kernel_option, geometry, staged,
                             staged_dtype, view_scale, nu, nv, n_views, source_distance,
                             detector_distance, detector_width, volume_width, rotation_count,
                             rotation_offset, volume, X, Y, Z, scale, accumulate, clip_negative,
@@ -1310,10 +1125,9 @@ int ctr_backproject_packed(void *stream, ctr_workspace *ws, int staged_dtype,
     CTR_REQUIRE(jobs, CTR_ERR_NULL_POINTER, "ctr_backproject_packed: jobs is NULL");
     CTR_REQUIRE(count >= 1 && count <= kMaxPacked, CTR_ERR_INVALID_ARGUMENT,
                 "ctr_backproject_packed: count %d outside [1, %d]", count, kMaxPacked);
-    CTR_REQUIRE(staged_dtype == CTR_F32 || staged_dtype == CTR_F16 || staged_dtype == CTR_BF16,
-                CTR_ERR_INVALID_ARGUMENT, "unknown staged dtype %d", staged_dtype);
+    CTR_REQUIRE(staged_dtype_valid(staged_dtype), CTR_ERR_INVALID_ARGUMENT,
+                "unknown staged dtype %d", staged_dtype);
     size_t total_views = 0, total_foot = 0;
-    bool packed = true;
     for (int q = 0; q < count; q++) {
         const ctr_backproject_job &j = jobs[q];
         total_foot += footprint_bytes(j.X, j.Z, j.n_views);
@@ -1356,20 +1170,16 @@ int ctr_backproject_packed(void *stream, ctr_workspace *ws, int staged_dtype,
         B.job[q] = make_job(j.staged, L, j.nu, j.nv, gp, j.view_scale, j.n_views, j.volume, j.X,
                             j.Y, j.Z, j.scale, j.accumulate, j.clip_negative);
         if ((rc = build_footprints(s, B.job[q], (float4 *)fp))) return rc;
-        packed = packed && f16_weights_fit(B.job[q], j.geometry, j.source_distance,
-                                           j.volume_width);
         B.job[q].block_begin = (int)blocks;
         blocks += (long long)B.job[q].gx * B.job[q].gy * ((j.Y + kTY - 1) / kTY);
         g += kGeomBytesPerView * (size_t)j.n_views;
         fp += footprint_bytes(j.X, j.Z, j.n_views);
     }
     CTR_REQUIRE(blocks < (1LL << 31), CTR_ERR_INVALID_ARGUMENT, "packed launch too large");
-    switch (lds_variant(workspace_option(w, CTR_OPTION_INTERPOLATION), staged_dtype, packed)) {
+    switch (staged_dtype) {
     case CTR_F32: launch_lds_packed<CTR_F32>(s, B, (unsigned)blocks); break;
     case CTR_F16: launch_lds_packed<CTR_F16>(s, B, (unsigned)blocks); break;
-    case CTR_BF16: launch_lds_packed<CTR_BF16>(s, B, (unsigned)blocks); break;
-    case kF16Packed: launch_lds_packed<kF16Packed>(s, B, (unsigned)blocks); break;
-    default: launch_lds_packed<kBF16Dot>(s, B, (unsigned)blocks); break;
+    default: launch_lds_packed<CTR_BF16>(s, B, (unsigned)blocks); break;
     }
     CTR_CHECK_LAUNCH();
     return CTR_OK;

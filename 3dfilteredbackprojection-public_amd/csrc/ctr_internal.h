@@ -66,6 +66,11 @@ inline int log2_exact(int n)
     return l;
 }
 
+// Staged texel formats: CTR_F32, or the 16-bit value-and-step texels of
+// CTR_F16 / CTR_BF16, which carry a per-view scale.
+constexpr bool staged_dtype_valid(int d) { return d == CTR_F32 || d == CTR_F16 || d == CTR_BF16; }
+constexpr bool staged_is_16bit(int d) { return d == CTR_F16 || d == CTR_BF16; }
+
 // Layout of the staged (zero-bordered, view-major) projections.
 struct StagedLayout {
     int pitch_u;   // texels per staged row

@@ -1134,22 +1134,58 @@ __device__ inline unsigned short f32_to_f16_bits(float f)
     return __builtin_bit_cast(unsigned short, h);
 }
 
-// fp32 staged -> 16-bit pair texels: texel c of a row holds
-// (q[c], q[c+1]) / view_scale, so one aligned 32-bit load gives both
-// horizontal bilinear taps.  One workgroup per staged row, four texels per
-// thread (pitch_u is a multiple of 32).  A view whose max is 0 gets scale 1.
-__device__ inline unsigned pack_pair(float a, float b, int bf16)
+// The upper 16 bits h of a texel whose lower 16 bits are `lo`, chosen so
+// that the whole 32-bit word read as an FP32 value is nearest to t: the
+// backprojector reads the step of a bf16 value-and-step texel that way,
+// without clearing the value's bits below it.  For a fixed `lo` the words
+// (h << 16) | lo are ordered like their values, so the nearest is h0 - 1, h0
+// or h0 + 1 with h0 the upper half of t; the error is at most half a bf16
+// step of t (NaN candidates never compare smaller).
+__device__ inline unsigned bf16_step_word(float t, unsigned lo)
 {
-    return bf16 ? (unsigned)f32_to_bf16_rne(a) | ((unsigned)f32_to_bf16_rne(b) << 16)
-                : (unsigned)f32_to_f16_bits(a) | ((unsigned)f32_to_f16_bits(b) << 16);
+    const unsigned h0 = __float_as_uint(t) >> 16;
+    unsigned best = (h0 << 16) | lo;
+    float err = fabsf(__uint_as_float(best) - t);
+#pragma unroll
+    for (int d = -1; d <= 1; d += 2) {
+        const unsigned c = (((h0 + (unsigned)d) & 0xffffu) << 16) | lo;
+        const float e = fabsf(__uint_as_float(c) - t);
+        if (e < err) {
+            best = c;
+            err = e;
+        }
+    }
+    return best;
 }
 
-__global__ __launch_bounds__(256) void quantize_pairs_kernel(const float *st, int pitch_u,
+// fp32 staged -> 16-bit value-and-step texels: texel (c, r) of a view holds
+// a value v = q[r][c] / view_scale rounded to 16 bits in its lower half and
+// the step d = q[r + 1][c] / view_scale - v in its upper half (q = 0 past
+// the view's last staged row), so one 32-bit load gives a column's two
+// vertical taps, v and v + d, and the error of the lower tap is the rounding
+// of the step alone.  fp16: d rounded to fp16.  bf16: the upper half is
+// chosen so that the whole word, read as FP32, is nearest to d
+// (bf16_step_word).  Row 0 (the zero border) holds (0, q[1][c]): the first
+// data row is its step.  One workgroup per staged row, four texels per
+// thread.
+__device__ inline unsigned pack_vstep(float v, float w, int bf16)
+{
+    if (bf16) {
+        const unsigned hv = f32_to_bf16_rne(v);
+        return bf16_step_word(w - __uint_as_float(hv << 16), hv);
+    }
+    const _Float16 hv = (_Float16)v;
+    return (unsigned)__builtin_bit_cast(unsigned short, hv) |
+           ((unsigned)f32_to_f16_bits(w - (float)hv) << 16);
+}
+
+__global__ __launch_bounds__(256) void quantize_vstep_kernel(const float *st, int pitch_u,
                                                              int pitch_v, float *view_scale,
                                                              unsigned *out, int bf16)
 {
     const unsigned row = blockIdx.x;  // staged row over all views
     const unsigned n = row / (unsigned)pitch_v;
+    const bool last = row % (unsigned)pitch_v == (unsigned)pitch_v - 1;
     float vs = view_scale[n];
     if (!(vs > 0.0f)) {
         vs = 1.0f;
@@ -1160,12 +1196,13 @@ __global__ __launch_bounds__(256) void quantize_pairs_kernel(const float *st, in
     unsigned *dst = out + (size_t)row * pitch_u;
     for (int c = 4 * threadIdx.x; c < pitch_u; c += 4 * blockDim.x) {
         const float4 a = *reinterpret_cast<const float4 *>(src + c);
-        const float e = c + 4 < pitch_u ? src[c + 4] : 0.0f;
+        const float4 b = last ? make_float4(0.0f, 0.0f, 0.0f, 0.0f)
+                              : *reinterpret_cast<const float4 *>(src + pitch_u + c);
         uint4 o;
-        o.x = pack_pair(a.x * inv, a.y * inv, bf16);
-        o.y = pack_pair(a.y * inv, a.z * inv, bf16);
-        o.z = pack_pair(a.z * inv, a.w * inv, bf16);
-        o.w = pack_pair(a.w * inv, e * inv, bf16);
+        o.x = pack_vstep(a.x * inv, b.x * inv, bf16);
+        o.y = pack_vstep(a.y * inv, b.y * inv, bf16);
+        o.z = pack_vstep(a.z * inv, b.z * inv, bf16);
+        o.w = pack_vstep(a.w * inv, b.w * inv, bf16);
         *reinterpret_cast<uint4 *>(dst + c) = o;
     }
 }
@@ -1190,7 +1227,7 @@ static int finish_staged(hipStream_t s, const float *st32, int nu, int nv, int n
     }
     const long long rows = (long long)L.pitch_v * n_views;
     CTR_REQUIRE(rows < (1LL << 31), CTR_ERR_INVALID_ARGUMENT, "too many staged rows");
-    quantize_pairs_kernel<<<(unsigned)rows, 256, 0, s>>>(st32, L.pitch_u, L.pitch_v, view_scale,
+    quantize_vstep_kernel<<<(unsigned)rows, 256, 0, s>>>(st32, L.pitch_u, L.pitch_v, view_scale,
                                                          (unsigned *)staged, dtype == CTR_BF16);
     CTR_CHECK_LAUNCH();
     return CTR_OK;
@@ -1538,8 +1575,8 @@ int ctr_filter_projections(void *stream, ctr_workspace *ws, const float *proj, i
                 "ctr_filter_projections: sizes (%d, %d, %d)", nu, nv, n_views);
     CTR_REQUIRE(layout == CTR_LAYOUT_PROJECTIONS || layout == CTR_LAYOUT_SINOGRAMS,
                 CTR_ERR_INVALID_ARGUMENT, "unknown layout %d", layout);
-    CTR_REQUIRE(staged_dtype == CTR_F32 || staged_dtype == CTR_F16 || staged_dtype == CTR_BF16,
-                CTR_ERR_INVALID_ARGUMENT, "unknown staged dtype %d", staged_dtype);
+    CTR_REQUIRE(staged_dtype_valid(staged_dtype), CTR_ERR_INVALID_ARGUMENT,
+                "unknown staged dtype %d", staged_dtype);
     CTR_REQUIRE(filter >= CTR_RAM_LAK && filter <= CTR_COSINE, CTR_ERR_INVALID_ARGUMENT,
                 "[CTAnalyzer Error] FBP2D::apply_filter() unknown filter type %d", filter);
     if (fdk_weight)
@@ -1615,8 +1652,8 @@ int ctr_stage_projections(void *stream, ctr_workspace *ws, const float *proj, in
                 "ctr_stage_projections: sizes (%d, %d, %d)", nu, nv, n_views);
     CTR_REQUIRE(layout == CTR_LAYOUT_PROJECTIONS || layout == CTR_LAYOUT_SINOGRAMS,
                 CTR_ERR_INVALID_ARGUMENT, "unknown layout %d", layout);
-    CTR_REQUIRE(staged_dtype == CTR_F32 || staged_dtype == CTR_F16 || staged_dtype == CTR_BF16,
-                CTR_ERR_INVALID_ARGUMENT, "unknown staged dtype %d", staged_dtype);
+    CTR_REQUIRE(staged_dtype_valid(staged_dtype), CTR_ERR_INVALID_ARGUMENT,
+                "unknown staged dtype %d", staged_dtype);
     ctr_workspace *w = nullptr;
     int rc = workspace_resolve(ws, &w);
     if (rc) return rc;

@@ -24,9 +24,6 @@ struct OptionDefaults {
     int value[CTR_OPTION_COUNT];
     OptionDefaults()
     {
-        value[CTR_OPTION_INTERPOLATION] = env_is("CTR_INTERPOLATION", "fast")
-                                              ? CTR_INTERP_FAST
-                                              : CTR_INTERP_FP32;
         value[CTR_OPTION_BACKPROJECT_KERNEL] =
             env_is("CTR_BACKPROJECT_KERNEL", "lds")      ? CTR_KERNEL_LDS
             : env_is("CTR_BACKPROJECT_KERNEL", "gather") ? CTR_KERNEL_GATHER
@@ -47,7 +44,6 @@ static const OptionDefaults g_option_defaults;
 static bool option_value_ok(int option, int value)
 {
     switch (option) {
-    case CTR_OPTION_INTERPOLATION: return value == CTR_INTERP_FP32 || value == CTR_INTERP_FAST;
     case CTR_OPTION_BACKPROJECT_KERNEL:
         return value == CTR_KERNEL_AUTO || value == CTR_KERNEL_LDS || value == CTR_KERNEL_GATHER;
     case CTR_OPTION_FORWARD_KERNEL: return value == CTR_KERNEL_AUTO || value == CTR_KERNEL_RAY;
@@ -227,8 +223,8 @@ int ctr_staged_layout(int nu, int nv, int n_views, int dtype, int *pitch_u,
     CTR_REQUIRE(nu > 0 && nv > 0 && n_views > 0, CTR_ERR_INVALID_ARGUMENT,
                 "ctr_staged_layout: sizes must be positive (%d, %d, %d)", nu,
                 nv, n_views);
-    CTR_REQUIRE(dtype == CTR_F32 || dtype == CTR_F16 || dtype == CTR_BF16,
-                CTR_ERR_INVALID_ARGUMENT, "unknown staged dtype %d", dtype);
+    CTR_REQUIRE(ctr::staged_dtype_valid(dtype), CTR_ERR_INVALID_ARGUMENT,
+                "unknown staged dtype %d", dtype);
     ctr::StagedLayout L = ctr::staged_layout(nu, nv, n_views, dtype);
     if (pitch_u) *pitch_u = L.pitch_u;
     if (pitch_v) *pitch_v = L.pitch_v;

@@ -6,10 +6,9 @@ ramp filter (FFT length 2048) into the staged layout, then the voxel-driven
 cone backprojection of all 720 views.  Inputs are synthetic: 30 seeded random
 ellipsoids rendered at 0.5 mm, projected by the ray-driven forward projector,
 Poisson noise at I0 = 1e5 and log.  The headline value is the FP16-stored
-form (filtered projections in FP16 with a per-view scale, FP32 interpolation
-and accumulation); the FP32-stored and BF16-stored forms are timed beside it,
-and the 16-bit forms once more with the 16-bit interpolation arithmetic
-(workspace option CTR_OPTION_INTERPOLATION = CTR_INTERP_FAST).
+form (filtered projections stored as FP16 value-and-step texels with a
+per-view scale, FP32 interpolation and accumulation); the FP32-stored and
+BF16-stored forms are timed beside it.
 
 Roofline of the dominant kernel (the backprojection).  It is bound by VALU
 issue, not HBM: its four bilinear taps per update come from LDS.  So
@@ -255,7 +254,6 @@ def main():
     log(f"workload {WORKLOAD}: {g.X}x{g.Y}x{g.Z} from {g.n_views} views of "
         f"{g.nu}x{g.nv}, FFT {FFT_LENGTH}")
     proj, fwd_ms = make_inputs(ctr, g, dev)
-    ws.set_interpolation(ctr.INTERP_FP32)
 
     log("timing FP32-stored form")
     f32_ms, f32_bp, vol32 = time_fdk(ctr, g, proj, ctr.F32, args.steps, args.warmup, ws, scale)
@@ -270,19 +268,6 @@ def main():
     b16_ms, b16_bp, volb = time_fdk(ctr, g, proj, ctr.BF16, max(3, args.steps // 2), 1, ws, scale)
     rel_rmse_bf = float((volb.double() - ref32.double()).norm() / ref32.double().norm())
     del volb
-    log("timing FP16 / BF16 with the 16-bit interpolation arithmetic")
-    ws.set_interpolation(ctr.INTERP_FAST)
-    fast = {}
-    for name, dt in (("fp16", ctr.F16), ("bf16", ctr.BF16)):
-        t_ms, t_bp, v = time_fdk(ctr, g, proj, dt, max(3, args.steps // 2), 1, ws, scale)
-        fast[name] = {
-            "gups": round(g.updates / (statistics.mean(t_ms) * 1e-3) / 1e9, 3),
-            "ms_per_step": round(statistics.mean(t_ms), 3),
-            "backproject_ms": round(statistics.mean(t_bp), 3),
-            "rel_rmse_vs_fp32": float((v.double() - ref32.double()).norm()
-                                      / ref32.double().norm())}
-        del v
-    ws.set_interpolation(ctr.INTERP_FP32)
     del ref32, vol32
     torch.cuda.synchronize()
 
@@ -380,7 +365,7 @@ def main():
         "data": "synthetic: 30 seeded ellipsoids, ray-driven projections, Poisson I0=1e5",
         "config": {"workload": "FDK cone 512x512x512 @0.5mm, 720 views/360deg, "
                                "768x768 panel @0.8mm, SOD 500, SDD 1000, FFT 2048, Ram-Lak",
-                   "storage": "fp16 + per-view scale"},
+                   "storage": "fp16 value-and-step texels + per-view scale"},
         "latency_ms": {"p50": round(statistics.median(f16_ms), 3),
                        "p99": round(float(np.percentile(f16_ms, 99)), 3)},
         "fp32_store": {"gups": round(gups32, 3), "ms_per_step": round(statistics.mean(f32_ms), 3),
@@ -393,8 +378,8 @@ def main():
         "fp16_vs_fp32_gups": round(gups / gups32, 3),
         "rel_rmse_fp16_vs_fp32": rel_rmse_16,
         "bf16_store": {"gups": round(g.updates / (statistics.mean(b16_ms) * 1e-3) / 1e9, 3),
+                       "backproject_ms": round(statistics.mean(b16_bp), 3),
                        "rel_rmse_vs_fp32": rel_rmse_bf},
-        "interp_16bit_arithmetic": fast,
         "backproject_gups_fp16": round(g.updates / (bp_ms * 1e-3) / 1e9, 3),
         "pcie_inclusive": {"h2d_ms": round(h2d_ms, 3),
                            "h2d_gb_per_s": round(h2d_gbps, 1),
@@ -411,8 +396,8 @@ def main():
             "frac_of_mix_ceiling": (round(achieved16 / 1024 / mix_ceiling, 4)
                                     if achieved16 and mix_ceiling else None),
             "valu_per_update": v16,
-            "kernel": "backproject_lds_kernel<fp16, 32> (FP32 interpolation; LDS 48 columns, "
-                      "pitch 51, 72 rows)",
+            "kernel": "backproject_lds_kernel<fp16, 32> (value-and-step texels, FP32 "
+                      "interpolation; LDS 48 columns, pitch 51, 72 rows)",
             "avg_ms": round(bp_ms, 3),
             "traffic": traffic16,
             "hbm_gbps": round(traffic16 / (bp_ms * 1e-3) / 1e9, 1) if traffic16 else None,

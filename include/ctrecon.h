@@ -45,21 +45,18 @@ extern "C" {
 #define CTR_ERR_NULL_POINTER (-5)
 #define CTR_ERR_OVER_BUDGET (-6)
 
-/* element types of staged (filtered) projections */
+/* element types of staged (filtered) projections: FP32, or 16-bit
+ * value-and-step texels with a per-view scale (layout below) */
 #define CTR_F32 0
 #define CTR_F16 1
 #define CTR_BF16 2
 
-/* workspace options (ctr_workspace_set_option) */
-#define CTR_OPTION_INTERPOLATION 0      /* CTR_INTERP_FP32 (default) or CTR_INTERP_FAST */
-#define CTR_OPTION_BACKPROJECT_KERNEL 1 /* CTR_KERNEL_AUTO (default), _LDS or _GATHER */
-#define CTR_OPTION_FORWARD_KERNEL 2     /* CTR_KERNEL_AUTO (default: slab) or _RAY */
-#define CTR_OPTION_FORWARD_ADDRESSING 3 /* CTR_KERNEL_AUTO (default) or _POINTER */
-#define CTR_OPTION_COUNT 4
-
-/* interpolation arithmetic of the 16-bit staged types */
-#define CTR_INTERP_FAST 0
-#define CTR_INTERP_FP32 1
+/* workspace options (ctr_workspace_set_option), all diagnostic kernel
+ * choices */
+#define CTR_OPTION_BACKPROJECT_KERNEL 0 /* CTR_KERNEL_AUTO (default), _LDS or _GATHER */
+#define CTR_OPTION_FORWARD_KERNEL 1     /* CTR_KERNEL_AUTO (default: slab) or _RAY */
+#define CTR_OPTION_FORWARD_ADDRESSING 2 /* CTR_KERNEL_AUTO (default) or _POINTER */
+#define CTR_OPTION_COUNT 3
 
 /* kernel choices of the diagnostic options */
 #define CTR_KERNEL_AUTO 0
@@ -111,19 +108,20 @@ int ctr_workspace_create(ctr_workspace **out);
 int ctr_workspace_destroy(ctr_workspace *ws);
 int ctr_workspace_reserve(ctr_workspace *ws, int n_views, int max_fft_length);
 
-/* Per-workspace options: the interpolation arithmetic of the 16-bit staged
- * forms (CTR_OPTION_INTERPOLATION) and diagnostic kernel choices.  Every
- * entry point reads the options of the workspace it is given when it
- * launches, so two threads driving two streams with two workspaces can use
- * different settings at the same time; a HIP graph keeps the kernels it
- * captured.  A new workspace -- and the per-device default workspace that
- * NULL selects -- starts from the environment as read once when the library
- * is loaded: CTR_INTERPOLATION=fast, CTR_BACKPROJECT_KERNEL=lds|gather,
+/* Per-workspace options: diagnostic kernel choices (which backprojection or
+ * forward-projection kernel runs, and the forward projector's addressing);
+ * results are the same under every choice, to the tolerances the kernels'
+ * tests state.  Every entry point reads the options of the workspace it is
+ * given when it launches, so two threads driving two streams with two
+ * workspaces can use different settings at the same time; a HIP graph keeps
+ * the kernels it captured.  A new workspace -- and the per-device default
+ * workspace that NULL selects -- starts from the environment as read once
+ * when the library is loaded: CTR_BACKPROJECT_KERNEL=lds|gather,
  * CTR_FORWARD_KERNEL=ray, CTR_FORWARD_ADDRESSING=pointer.  The default
  * workspace is shared by every NULL caller on a device, so its options
  * cannot be changed (CTR_ERR_INVALID_ARGUMENT for ws == NULL).
  * ctr_workspace_set_option returns the previous value.  No reference
- * counterpart (its sampler interpolated FP32 texels; one GL context). */
+ * counterpart (one GL context, one set of shaders). */
 int ctr_workspace_set_option(ctr_workspace *ws, int option, int value);
 /* The option's value for ws (NULL: the default workspaces' value, which is
  * the one read from the environment at library load). */
@@ -194,8 +192,9 @@ int ctr_filter_rows(void *stream, const float *in, float *out, int width,
  * layout the backprojector reads (below).  fdk_weight != 0 multiplies each
  * sample by SDD / sqrt(SDD^2 + u^2 + v^2) (FP64) before filtering
  * (FBP3D.cpp:369-387).  For CTR_F16 / CTR_BF16 every view is scaled by its
- * max |value| (stored in view_scale[N], device memory) before rounding; for
- * CTR_F32 view_scale may be NULL.  `layout` gives the input layout. */
+ * max |value| (stored in view_scale[N], device memory; 1 for a view that is
+ * all zero) before rounding; for CTR_F32 view_scale may be NULL.  `layout`
+ * gives the input layout. */
 int ctr_filter_projections(void *stream, ctr_workspace *ws, const float *proj,
                            int layout, int nu, int nv, int n_views,
                            int fft_length, int filter, int fdk_weight,
@@ -209,10 +208,15 @@ int ctr_filter_projections(void *stream, ctr_workspace *ws, const float *proj,
  *   staged[(n * pitch_v + iv + 1) * pitch_u + iu + 1],
  * with one zero column/row before the data, two zero rows and at least two
  * zero columns after it; pitch_u is a multiple of 32 texels (128-byte rows).
- * For CTR_F16 and
- * CTR_BF16 each 32-bit texel holds the pair (q[iu], q[iu+1]) / view_scale[n]
- * so one aligned load feeds both horizontal taps.  No reference counterpart
- * (the reference sampled its Texture2DArray directly, FBP3D.cpp:543). */
+ * For CTR_F16 and CTR_BF16 each 32-bit texel at staged (c, r) is a
+ * value-and-step texel: its lower 16 bits hold v = q[r][c] / view_scale[n]
+ * rounded to the 16-bit type, its upper 16 bits the step d to the next
+ * staged row, q[r + 1][c] / view_scale[n] - v, so one load gives a column's
+ * two vertical taps v and v + d (q taken as 0 past the view's last staged
+ * row; the border row r = 0 therefore holds (0, first data row)).  FP16:
+ * d rounded to FP16.  BF16: the upper half is chosen so that the whole word
+ * read as an FP32 value is nearest to d.  No reference counterpart (the
+ * reference sampled its Texture2DArray directly, FBP3D.cpp:543). */
 int ctr_staged_layout(int nu, int nv, int n_views, int dtype, int *pitch_u,
                       int *pitch_v, size_t *bytes);
 /* Copies plain projections (either layout) into the staged layout,
@@ -237,20 +241,11 @@ int ctr_stage_projections(void *stream, ctr_workspace *ws, const float *proj,
  * bilinear is zero outside the detector.  clip_negative != 0 then applies
  * max(v, 0) (FBP3D.cpp:444-458, run after every reference 3D
  * backprojection).  view_scale may be NULL for CTR_F32.  Coordinates are set
- * up in FP64 per column and view and stepped in FP32 along y; accumulation
- * is FP32.  Interpolation is FP32 (16-bit pairs widened exactly) under
- * CTR_INTERP_FP32, the default CTR_OPTION_INTERPOLATION of the workspace,
- * and for every format on volumes under 32 voxels along y.  Under
- * CTR_INTERP_FAST the
- * 16-bit formats use their own arithmetic for the interpolation weights:
- * CTR_F16 pairs are interpolated in packed f16 (per-sample relative error
- * ~2^-11; about 4e-4 rel RMSE on white-noise projections, 4e-5 on the 512^3
- * phantom scan, against 2e-5 with FP32 interpolation) wherever the distance weight (SOD / (SOD - r))^2 of the volume's half
- * diagonal r stays below 1024, in FP32 otherwise; CTR_BF16 pairs are
- * weighted with bf16 column weights by v_dot2_f32_bf16 with FP32 products
- * and sums (about 1.7e-3 rel RMSE on white-noise projections, 2.5e-4 on the
- * 512^3 phantom scan, against 1.6e-4 with FP32 weights).  The sum over
- * views runs in view order, so results are reproducible bit for bit. */
+ * up in FP64 per column and view and stepped in FP32 along y; interpolation
+ * and accumulation are FP32 in every format (16-bit texels widened
+ * exactly): for CTR_F16 / CTR_BF16 the vertical taps of a column are v and
+ * v + d of its value-and-step texel.  The sum over views runs in view
+ * order, so results are reproducible bit for bit. */
 int ctr_backproject(void *stream, ctr_workspace *ws, int geometry,
                     const void *staged, int staged_dtype,
                     const float *view_scale, int nu, int nv, int n_views,

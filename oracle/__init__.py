@@ -62,6 +62,12 @@ def lib():
                                                   _i]
         L.orc_backproject_parallel.argtypes = [_p, _i, _i, _i, _d, _d, _d, _d,
                                                _d, _i, _i, _i, _p, _i]
+        L.orc_backproject_cone_cells.argtypes = [_p, _p, _i, _i, _i, _d, _d, _d,
+                                                 _d, _d, _d, _d, _i, _i, _i, _p,
+                                                 _i]
+        L.orc_backproject_parallel_cells.argtypes = [_p, _p, _i, _i, _i, _d, _d,
+                                                     _d, _d, _d, _i, _i, _i, _p,
+                                                     _i]
         L.orc_backproject_cone_f32.argtypes = [_p, _i, _i, _i, _d, _d, _d, _d,
                                                _d, _d, _d, _i, _i, _i, _i, _i,
                                                _p, _i]
@@ -282,6 +288,39 @@ def backproject_parallel(proj, detector_width, volume_width, rotation, offset,
     lib().orc_backproject_parallel(_ptr(proj), nu, nv, n, detector_width,
                                    volume_width, rotation, offset, scale, X, Y,
                                    Z, _ptr(vol), threads or threads_default())
+    return vol
+
+
+def backproject_cone_cells(top, bot, sod, odd, detector_width, volume_width,
+                           rotation, offset, scale, shape, threads=None):
+    """Cone backprojection of views given per interpolation cell: top and
+    bot (N, nv + 1, nu) hold, for cell c = iv + 1, detector row iv and row
+    iv + 1 as that cell sees them (the dequantized 16-bit value-and-step
+    texels).  Returns the (Z, Y, X) float64 volume."""
+    top, bot = _f32(top), _f32(bot)
+    assert top.shape == bot.shape
+    n, cells, nu = top.shape
+    X, Y, Z = shape
+    vol = np.zeros((Z, Y, X), dtype=np.float64)
+    lib().orc_backproject_cone_cells(_ptr(top), _ptr(bot), nu, cells - 1, n, sod,
+                                     odd, detector_width, volume_width, rotation,
+                                     offset, scale, X, Y, Z, _ptr(vol),
+                                     threads or threads_default())
+    return vol
+
+
+def backproject_parallel_cells(top, bot, detector_width, volume_width,
+                               rotation, offset, scale, shape, threads=None):
+    """Parallel-beam counterpart of backproject_cone_cells."""
+    top, bot = _f32(top), _f32(bot)
+    assert top.shape == bot.shape
+    n, cells, nu = top.shape
+    X, Y, Z = shape
+    vol = np.zeros((Z, Y, X), dtype=np.float64)
+    lib().orc_backproject_parallel_cells(_ptr(top), _ptr(bot), nu, cells - 1, n,
+                                         detector_width, volume_width, rotation,
+                                         offset, scale, X, Y, Z, _ptr(vol),
+                                         threads or threads_default())
     return vol
 
 

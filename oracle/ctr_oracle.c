@@ -310,13 +310,49 @@ static double bilinear0(const float *img, int nu, int nv, double fu, double fv)
     return acc;
 }
 
+/* Bilinear sample of a view given per cell: cell c = iv + 1 (iv = -1 ..
+ * nv - 1) spans detector rows iv and iv + 1, with top[c] the values of row
+ * iv and bot[c] the values of row iv + 1 as that cell sees them ((nv + 1) x
+ * nu arrays each).  Zero outside the detector as bilinear0.  With
+ * top[c] = row c - 1 and bot[c] = row c (zero rows outside) it is bilinear0
+ * up to the order of the FP64 sum. */
+static double bilinear_cells(const float *top, const float *bot, int nu, int nv,
+                             double fu, double fv)
+{
+    if (!(fu > -1.0 && fu < (double)nu && fv > -1.0 && fv < (double)nv))
+        return 0.0;
+    double fu0 = floor(fu), fv0 = floor(fv);
+    int iu = (int)fu0, c = (int)fv0 + 1;
+    double wu = fu - fu0, wv = fv - fv0;
+    const float *rt = top + (size_t)c * nu, *rb = bot + (size_t)c * nu;
+    double acc = 0.0;
+    for (int a = 0; a < 2; a++) {
+        int col = iu + a;
+        if (col < 0 || col >= nu) continue;
+        double wc = a ? wu : 1.0 - wu;
+        acc += wc * ((1.0 - wv) * (double)rt[col] + wv * (double)rb[col]);
+    }
+    return acc;
+}
+
+/* One view's sample: bilinear0 of proj_n, or bilinear_cells of the n-th
+ * (top, bot) pair when bot is given. */
+static double sample_view(const float *proj, const float *bot, int nu, int nv, int n,
+                          double fu, double fv)
+{
+    if (!bot)
+        return bilinear0(proj + (size_t)n * nu * nv, nu, nv, fu, fv);
+    const size_t cells = (size_t)(nv + 1) * nu;
+    return bilinear_cells(proj + (size_t)n * cells, bot + (size_t)n * cells, nu, nv, fu, fv);
+}
+
 static double view_angle(int n, int n_views, double rot, double offset)
 {
     /* FBP3D.cpp:536-540 */
     return 2.0 * M_PI * rot * (double)n / (double)n_views + offset;
 }
 
-void orc_backproject_cone_region(const float *proj, int nu, int nv,
+static void backproject_cone_box(const float *proj, const float *bot, int nu, int nv,
                                  int n_views, double sod, double odd,
                                  double detector_width, double volume_width,
                                  double rotation_count, double rotation_offset,
@@ -333,7 +369,6 @@ void orc_backproject_cone_region(const float *proj, int nu, int nv,
         for (int n = 0; n < n_views; n++) {
             double th = view_angle(n, n_views, rotation_count, rotation_offset);
             double c = cos(th), s = sin(th);
-            const float *pv = proj + (size_t)n * nu * nv;
             for (int j = j0; j < j1; j++) {
                 double y = (j + 0.5 - 0.5 * Y) * d;
                 for (int i = i0; i < i1; i++) {
@@ -346,11 +381,36 @@ void orc_backproject_cone_region(const float *proj, int nu, int nv,
                     double fv = v / delta + 0.5 * nv - 0.5;
                     double w = sod / L;
                     out[((size_t)(k - k0) * nj + (j - j0)) * ni + (i - i0)] +=
-                        scale * w * w * bilinear0(pv, nu, nv, fu, fv);
+                        scale * w * w * sample_view(proj, bot, nu, nv, n, fu, fv);
                 }
             }
         }
     }
+}
+
+void orc_backproject_cone_region(const float *proj, int nu, int nv,
+                                 int n_views, double sod, double odd,
+                                 double detector_width, double volume_width,
+                                 double rotation_count, double rotation_offset,
+                                 double scale, int X, int Y, int Z, int i0,
+                                 int i1, int j0, int j1, int k0, int k1,
+                                 double *out, int threads)
+{
+    backproject_cone_box(proj, NULL, nu, nv, n_views, sod, odd, detector_width,
+                         volume_width, rotation_count, rotation_offset, scale, X,
+                         Y, Z, i0, i1, j0, j1, k0, k1, out, threads);
+}
+
+void orc_backproject_cone_cells(const float *top, const float *bot, int nu,
+                                int nv, int n_views, double sod, double odd,
+                                double detector_width, double volume_width,
+                                double rotation_count, double rotation_offset,
+                                double scale, int X, int Y, int Z, double *vol,
+                                int threads)
+{
+    backproject_cone_box(top, bot, nu, nv, n_views, sod, odd, detector_width,
+                         volume_width, rotation_count, rotation_offset, scale, X,
+                         Y, Z, 0, X, 0, Y, 0, Z, vol, threads);
 }
 
 void orc_backproject_cone(const float *proj, int nu, int nv, int n_views,
@@ -365,11 +425,11 @@ void orc_backproject_cone(const float *proj, int nu, int nv, int n_views,
                                 Z, vol, threads);
 }
 
-void orc_backproject_parallel(const float *proj, int nu, int nv, int n_views,
-                              double detector_width, double volume_width,
-                              double rotation_count, double rotation_offset,
-                              double scale, int X, int Y, int Z, double *vol,
-                              int threads)
+static void backproject_parallel_all(const float *proj, const float *bot, int nu, int nv,
+                                     int n_views, double detector_width,
+                                     double volume_width, double rotation_count,
+                                     double rotation_offset, double scale, int X,
+                                     int Y, int Z, double *vol, int threads)
 {
     double delta = detector_width / nu, d = volume_width / X;
     set_threads(threads);
@@ -379,7 +439,6 @@ void orc_backproject_parallel(const float *proj, int nu, int nv, int n_views,
         for (int n = 0; n < n_views; n++) {
             double th = view_angle(n, n_views, rotation_count, rotation_offset);
             double c = cos(th), s = sin(th);
-            const float *pv = proj + (size_t)n * nu * nv;
             for (int j = 0; j < Y; j++) {
                 double y = (j + 0.5 - 0.5 * Y) * d;
                 for (int i = 0; i < X; i++) {
@@ -388,11 +447,33 @@ void orc_backproject_parallel(const float *proj, int nu, int nv, int n_views,
                     double fu = sl / delta + 0.5 * nu - 0.5;
                     double fv = y / delta + 0.5 * nv - 0.5;
                     vol[((size_t)k * Y + j) * X + i] +=
-                        scale * bilinear0(pv, nu, nv, fu, fv);
+                        scale * sample_view(proj, bot, nu, nv, n, fu, fv);
                 }
             }
         }
     }
+}
+
+void orc_backproject_parallel(const float *proj, int nu, int nv, int n_views,
+                              double detector_width, double volume_width,
+                              double rotation_count, double rotation_offset,
+                              double scale, int X, int Y, int Z, double *vol,
+                              int threads)
+{
+    backproject_parallel_all(proj, NULL, nu, nv, n_views, detector_width,
+                             volume_width, rotation_count, rotation_offset, scale,
+                             X, Y, Z, vol, threads);
+}
+
+void orc_backproject_parallel_cells(const float *top, const float *bot, int nu,
+                                    int nv, int n_views, double detector_width,
+                                    double volume_width, double rotation_count,
+                                    double rotation_offset, double scale, int X,
+                                    int Y, int Z, double *vol, int threads)
+{
+    backproject_parallel_all(top, bot, nu, nv, n_views, detector_width,
+                             volume_width, rotation_count, rotation_offset, scale,
+                             X, Y, Z, vol, threads);
 }
 
 void orc_backproject_cone_f32(const float *proj, int nu, int nv, int n_views,

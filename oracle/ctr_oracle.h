@@ -119,6 +119,25 @@ void orc_backproject_parallel(const float *proj, int nu, int nv, int n_views,
                               double rotation_count, double rotation_offset,
                               double scale, int X, int Y, int Z, double *vol,
                               int threads);
+/* The same sums with each view given per interpolation cell: top and bot
+ * are (n_views, nv + 1, nu); cell c = iv + 1 (iv = -1 .. nv - 1) spans
+ * detector rows iv and iv + 1, top[n][c] holding row iv and bot[n][c] row
+ * iv + 1 as that cell sees them.  This is the dequantized model of the
+ * backprojector's 16-bit value-and-step texels (row iv + 1 seen from cell
+ * iv is value + step of row iv, not row iv + 1's own value).  With
+ * top[n][c] = row c - 1 and bot[n][c] = row c (zero outside the detector)
+ * it is orc_backproject_cone / _parallel. */
+void orc_backproject_cone_cells(const float *top, const float *bot, int nu,
+                                int nv, int n_views, double sod, double odd,
+                                double detector_width, double volume_width,
+                                double rotation_count, double rotation_offset,
+                                double scale, int X, int Y, int Z, double *vol,
+                                int threads);
+void orc_backproject_parallel_cells(const float *top, const float *bot, int nu,
+                                    int nv, int n_views, double detector_width,
+                                    double volume_width, double rotation_count,
+                                    double rotation_offset, double scale, int X,
+                                    int Y, int Z, double *vol, int threads);
 /* FP32 voxel loop with FP64 per-view geometry: the CPU baseline timing path
  * (same conventions, float accumulation). */
 void orc_backproject_cone_f32(const float *proj, int nu, int nv, int n_views,

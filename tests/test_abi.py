@@ -64,6 +64,13 @@ def test_argument_errors_without_device(ctr):
                                    1.0, 0.0, fake, 0, 8, 8, 4, 1.0) == -1
     assert lib.ctr_render_ellipsoids(None, None, fake, 5000, fake, 4, 4, 4, 4.0) == -1
     assert lib.ctr_staged_layout(0, 4, 4, 0, None, None, None) == -1
+    # staged formats are CTR_F32, CTR_F16 and CTR_BF16 only
+    for dt in (3, 4, -1):
+        assert lib.ctr_staged_layout(8, 8, 4, dt, None, None, None) == -1
+        assert "unknown staged dtype" in ctr.last_error()
+        assert lib.ctr_backproject(None, None, 0, fake, dt, fake, 8, 8, 4, 100.0, 1.0, 10.0,
+                                   10.0, 1.0, 0.0, fake, 8, 8, 8, 1.0, 0, 0) == -1
+        assert "unknown staged dtype" in ctr.last_error()
 
 
 def test_staged_layout_sizes(ctr):
@@ -84,28 +91,26 @@ def test_workspace_options_without_device(ctr):
     """The default workspace (NULL) reports the load-time options and refuses
     changes; unknown options and values are rejected; no device is touched."""
     lib = ctr.lib()
-    assert lib.ctr_workspace_get_option(None, ctr.OPTION_INTERPOLATION) in (
-        ctr.INTERP_FAST, ctr.INTERP_FP32)
-    assert lib.ctr_workspace_set_option(None, ctr.OPTION_INTERPOLATION,
-                                        ctr.INTERP_FAST) == -1
+    assert lib.ctr_workspace_get_option(None, ctr.OPTION_BACKPROJECT_KERNEL) in (
+        ctr.KERNEL_AUTO, ctr.KERNEL_LDS, ctr.KERNEL_GATHER)
+    assert lib.ctr_workspace_set_option(None, ctr.OPTION_BACKPROJECT_KERNEL,
+                                        ctr.KERNEL_GATHER) == -1
     assert "default workspace" in ctr.last_error()
-    assert lib.ctr_workspace_get_option(None, 9) == -1
-    assert "unknown option" in ctr.last_error()
-    assert lib.ctr_workspace_get_option(None, -1) == -1
+    for bad in (3, 9, -1):
+        assert lib.ctr_workspace_get_option(None, bad) == -1
+        assert "unknown option" in ctr.last_error()
 
 
 @pytest.mark.parametrize("env,option,value", [
-    ({"CTR_INTERPOLATION": "fp32"}, 0, 1), ({"CTR_INTERPOLATION": "fast"}, 0, 0),
-    ({"CTR_INTERPOLATION": ""}, 0, 1),
-    ({"CTR_BACKPROJECT_KERNEL": "lds"}, 1, 1), ({"CTR_BACKPROJECT_KERNEL": "gather"}, 1, 2),
-    ({"CTR_FORWARD_KERNEL": "ray"}, 2, 3), ({"CTR_FORWARD_ADDRESSING": "pointer"}, 3, 4),
-    ({}, 1, 0), ({}, 2, 0), ({}, 3, 0)])
+    ({"CTR_BACKPROJECT_KERNEL": "lds"}, 0, 1), ({"CTR_BACKPROJECT_KERNEL": "gather"}, 0, 2),
+    ({"CTR_BACKPROJECT_KERNEL": ""}, 0, 0),
+    ({"CTR_FORWARD_KERNEL": "ray"}, 1, 3), ({"CTR_FORWARD_ADDRESSING": "pointer"}, 2, 4),
+    ({}, 0, 0), ({}, 1, 0), ({}, 2, 0)])
 def test_options_read_from_environment_at_load(ctr, env, option, value):
     """The environment is read once, when the library is loaded: changing it
     afterwards changes nothing."""
     code = ("import ctypes, os, sys; L = ctypes.CDLL(sys.argv[1]); "
             "a = L.ctr_workspace_get_option(None, int(sys.argv[2])); "
-            "os.environ['CTR_INTERPOLATION'] = 'fast'; "
             "os.environ['CTR_BACKPROJECT_KERNEL'] = 'gather'; "
             "os.environ['CTR_FORWARD_KERNEL'] = 'ray'; "
             "os.environ['CTR_FORWARD_ADDRESSING'] = 'pointer'; "

@@ -1,12 +1,9 @@
 """Accuracy of each workload's reconstruction against the FP64 oracle pipeline
 on the same inputs: relative RMSE, max-abs error, max-rel error (over
 max |ref|) and PSNR = 20 log10(max |ref| / RMSE), for the FP32 form and the
-FP16 / BF16 storage forms (filtered projections stored in 16 bits with a
-per-view scale; FP32 accumulation), each 16-bit form under both
-interpolation modes: FP32 interpolation, the default ("fp16", "bf16"), and
-the 16-bit interpolation arithmetic ("fp16/fast", "bf16/fast").  Errors are
-taken
-before the negative clip.  The oracle pipeline is the FP64 restatement:
+FP16 / BF16 storage forms (filtered projections stored as 16-bit
+value-and-step texels with a per-view scale; FP32 interpolation and
+accumulation).  Errors are taken before the negative clip.  The oracle pipeline is the FP64 restatement:
 FDK weight, reference radix-2 ramp filter and voxel loop in FP64, with FP32
 arrays between the stages (the same interchange format the product uses).
 
@@ -47,21 +44,7 @@ def check(case, name, m):
 
 
 def forms(ctr):
-    return (("fp32", ctr.F32), ("fp16", ctr.F16), ("bf16", ctr.BF16),
-            ("fp16/fast", ctr.F16), ("bf16/fast", ctr.BF16))
-
-
-_WS = {}
-
-
-def mode(ctr, name):
-    """Workspace whose interpolation arithmetic is the form's (FP32 for the
-    plain forms, the 16-bit arithmetic for the /fast forms)."""
-    fast = name.endswith("fast")
-    if fast not in _WS:
-        _WS[fast] = ctr.Workspace()
-        _WS[fast].set_interpolation(ctr.INTERP_FAST if fast else ctr.INTERP_FP32)
-    return _WS[fast]
+    return (("fp32", ctr.F32), ("fp16", ctr.F16), ("bf16", ctr.BF16))
 
 
 def _t(dev, a):
@@ -79,10 +62,8 @@ def test_accuracy_par2d_256(ctr, orc, dev):
     ref = orc.backproject_parallel(q, g.detector_width, g.volume_width, g.rotation_count,
                                    0.0, scale, (g.X, 1, g.Z))
     for name, dt in forms(ctr):
-        ws = mode(ctr, name)
-        staged, vs = ctr.filter_projections(_t(dev, sino), g, ctr.RAM_LAK, nfft, dt, fdk=False,
-                                            ws=ws)
-        img = ctr.backproject(staged, g, dt, view_scale=vs, scale=scale, ws=ws).cpu().numpy()
+        staged, vs = ctr.filter_projections(_t(dev, sino), g, ctr.RAM_LAK, nfft, dt, fdk=False)
+        img = ctr.backproject(staged, g, dt, view_scale=vs, scale=scale).cpu().numpy()
         check("par2d_256", name, metrics(img, ref))
 
 
@@ -98,9 +79,8 @@ def test_accuracy_fan2d_512(ctr, orc, dev):
     ref = orc.backproject_cone(q, g.source_distance, g.detector_distance, g.detector_width,
                                g.volume_width, 1.0, 0.0, scale, (g.X, 1, g.Z))
     for name, dt in forms(ctr):
-        ws = mode(ctr, name)
-        staged, vs = ctr.filter_projections(sino, g, ctr.COSINE, nfft, dt, fdk=True, ws=ws)
-        rec = ctr.backproject(staged, g, dt, view_scale=vs, scale=scale, ws=ws).cpu().numpy()
+        staged, vs = ctr.filter_projections(sino, g, ctr.COSINE, nfft, dt, fdk=True)
+        rec = ctr.backproject(staged, g, dt, view_scale=vs, scale=scale).cpu().numpy()
         check("fan2d_512", name, metrics(rec, ref))
 
 
@@ -121,9 +101,8 @@ def test_accuracy_fdk_512(ctr, orc, dev):
              ((80, 112), (440, 472), (128, 160)))    # off axis, high cone angle
     got = {}
     for name, dt in forms(ctr):
-        ws = mode(ctr, name)
-        staged, vs = ctr.filter_projections(proj, g, ctr.RAM_LAK, nfft, dt, fdk=True, ws=ws)
-        rec = ctr.backproject(staged, g, dt, view_scale=vs, scale=scale, ws=ws)
+        staged, vs = ctr.filter_projections(proj, g, ctr.RAM_LAK, nfft, dt, fdk=True)
+        rec = ctr.backproject(staged, g, dt, view_scale=vs, scale=scale)
         got[name] = [rec[k0:k1, j0:j1, i0:i1].cpu().numpy()
                      for (i0, i1), (j0, j1), (k0, k1) in boxes]
         del staged, rec

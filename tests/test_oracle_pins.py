@@ -267,3 +267,62 @@ def test_analytic_cone_sphere_closed_form(orc):
         ref = rho * 2 * math.sqrt(max(r * r - h2, 0.0))
         assert p[n, iv, iu] == pytest.approx(ref, abs=1e-9)
     assert p.max() > 2 * rho * r * 0.9
+
+
+# --- per-cell backprojection (the dequantized model of the 16-bit
+#     value-and-step texels) ------------------------------------------------
+
+def _plain_as_cells(q):
+    """Cells of a plain view stack q (N, nv, nu): cell c = iv + 1 holds row
+    iv on top and row iv + 1 below, rows outside the detector zero."""
+    n, nv, nu = q.shape
+    z = np.zeros((n, 1, nu), dtype=np.float32)
+    return np.concatenate([z, q], axis=1), np.concatenate([q, z], axis=1)
+
+
+def test_cells_backprojection_reduces_to_plain(orc):
+    """With each cell's rows taken from one view stack, the per-cell
+    backprojection is the pinned plain one (cone and parallel), up to the
+    order of the FP64 sum."""
+    rng = np.random.default_rng(31)
+    q = rng.standard_normal((40, 21, 26)).astype(np.float32)
+    top, bot = _plain_as_cells(q)
+    args = (150.0, 90.0, 52.0, 20.0, 1.0, 0.3, 0.7, (20, 18, 22))
+    a = orc.backproject_cone_cells(top, bot, *args)
+    b = orc.backproject_cone(q, *args)
+    assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+    pargs = (26.0, 20.0, 0.5, 0.1, 0.7, (20, 18, 22))
+    a = orc.backproject_parallel_cells(top, bot, *pargs)
+    b = orc.backproject_parallel(q, *pargs)
+    assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+
+
+def test_cells_take_the_bottom_row_from_bot(orc):
+    """A cell's lower row comes from `bot`, not from the next cell's top: with
+    bot = top + 1 everywhere (and top from a plain stack), every sample gains
+    exactly its vertical weight wv, so the volume gains the backprojection of
+    the stack whose cells are all (0, 1).  Adding 1 to top instead gains the
+    stack whose cells are all (1, 0); the two gains add up to the all-ones
+    stack, which inside the detector is the plain backprojection of ones."""
+    rng = np.random.default_rng(32)
+    n, nv, nu = 30, 17, 23
+    # multiples of 1/64: q + 1 is exact in float32
+    q = (np.round(rng.standard_normal((n, nv, nu)) * 64) / 64).astype(np.float32)
+    top, bot = _plain_as_cells(q)
+    ones = np.ones_like(top)
+    args = (150.0, 90.0, 46.0, 16.0, 1.0, 0.2, 0.9, (16, 14, 16))
+    base = orc.backproject_cone_cells(top, bot, *args)
+    gain = orc.backproject_cone_cells(top, bot + ones, *args) - base
+    wv_only = orc.backproject_cone_cells(np.zeros_like(top), ones, *args)
+    assert np.abs(gain - wv_only).max() <= 1e-12 * np.abs(wv_only).max()
+    assert np.abs(wv_only).max() > 0.05
+    swapped = orc.backproject_cone_cells(top + ones, bot, *args) - base
+    top_only = orc.backproject_cone_cells(ones, np.zeros_like(top), *args)
+    assert np.abs(swapped - top_only).max() <= 1e-12 * np.abs(top_only).max()
+    # wv and 1 - wv partition the unit cell: (1, 1) cells give the all-ones
+    # stack's backprojection (interior columns and rows)
+    full = orc.backproject_cone_cells(ones, ones, *args)
+    plain = orc.backproject_cone(np.ones((n, nv, nu), np.float32), *args)
+    inner = (slice(4, 12), slice(4, 10), slice(4, 12))
+    assert np.allclose((wv_only + top_only)[inner], full[inner], rtol=1e-12, atol=0)
+    assert np.allclose(full[inner], plain[inner], rtol=1e-12, atol=0)
