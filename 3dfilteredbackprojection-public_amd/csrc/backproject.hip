@@ -109,12 +109,13 @@ int check_geometry(int geometry, double sod, double odd, double detector_width,
     return CTR_OK;
 }
 
-// Detector footprint of every 16 x 16 column tile of the LDS kernel, per
+// Detector footprint of every tile x tile column tile of the LDS kernel, per
 // view, from the tile's 4 corner columns in FP64: (min su, max su, min 1/den,
 // max 1/den) in staged coordinates.  For a run of rows j the staged v range
 // follows as cv + 1 + Vj (j + oy) [min, max] 1/den, because
 // fv = cv + Vj (j + oy) / den for this parameterisation (cone and parallel).
-__global__ void footprint_kernel(const double *g64, int n_views, int gx, int gy, float4 *out)
+__global__ void footprint_kernel(const double *g64, int n_views, int gx, int gy, int tile,
+                                 float4 *out)
 {
     long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long total = (long long)n_views * gx * gy;
@@ -126,7 +127,7 @@ __global__ void footprint_kernel(const double *g64, int n_views, int gx, int gy,
     double umin = 1e300, umax = -1e300, imin = 1e300, imax = -1e300;
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b) {
-            const double ci = bx * 16 + 15 * a, ck = by * 16 + 15 * b;
+            const double ci = bx * tile + (tile - 1) * a, ck = by * tile + (tile - 1) * b;
             const double inv = 1.0 / fma(g[7], ci, fma(g[8], ck, g[9]));
             const double su = fma(g[0], ci, fma(g[1], ck, g[2])) * inv + 1.0;
             umin = fmin(umin, su);
@@ -490,13 +491,23 @@ __global__ __launch_bounds__(256) void backproject_kernel(BpJob J)
 #endif
 constexpr int kRectChunk = 128; // views per refill of the rectangle table
 
+// Column tile edge of the LDS kernel per format: a tile of T x T columns is
+// (T / 8)^2 waves, each owning an 8 x 8 patch.
+#ifndef CTR_LDS_TILE32
+#define CTR_LDS_TILE32 16
+#endif
+#ifndef CTR_LDS_TILE16
+#define CTR_LDS_TILE16 16
+#endif
 template <int DT>
 struct LdsConfig {
-    static constexpr int kCols = CTR_LDS_COLS16, kPitch = CTR_LDS_PITCH16, kRows = CTR_LDS_ROWS16;
+    static constexpr int kCols = CTR_LDS_COLS16, kPitch = CTR_LDS_PITCH16, kRows = CTR_LDS_ROWS16,
+                         kTile = CTR_LDS_TILE16;
 };
 template <>
 struct LdsConfig<CTR_F32> {
-    static constexpr int kCols = CTR_LDS_COLS32, kPitch = CTR_LDS_PITCH32, kRows = CTR_LDS_ROWS32;
+    static constexpr int kCols = CTR_LDS_COLS32, kPitch = CTR_LDS_PITCH32, kRows = CTR_LDS_ROWS32,
+                         kTile = CTR_LDS_TILE32;
 };
 
 enum RectMode { RECT_SKIP = 0, RECT_LDS = 1, RECT_DIRECT = 2 };
@@ -510,12 +521,14 @@ __device__ inline int rect_cols(const int4 &r) { return r.w >> 8; }
 template <int DT>
 struct LdsLayout {
     static constexpr int kCols = LdsConfig<DT>::kCols, kPitch = LdsConfig<DT>::kPitch,
-                         kRows = LdsConfig<DT>::kRows;
+                         kRows = LdsConfig<DT>::kRows, kTile = LdsConfig<DT>::kTile;
+    static constexpr int kWaves = (kTile / 8) * (kTile / 8), kThreads = 64 * kWaves;
     static constexpr int kRectBytes = (kRectChunk + 1) * 16;
     static constexpr int kBufDwords = kRows * kPitch;
     static constexpr int kDwords = kRectBytes / 4 + 2 * kBufDwords;
     static_assert((2 * kBufDwords) % 4 == 0, "buffers are cleared as uint4");
-    static_assert(kCols <= 64 && kPitch >= kCols + 1 && kPitch + 1 <= 255, "ds_read2 offsets");
+    static_assert(kPitch >= kCols + 1 && kPitch + 1 <= 255, "ds_read2 offsets");
+    static_assert(kTile % 8 == 0 && kThreads <= 1024 && kThreads > kRectChunk, "tile");
 };
 
 // fp32 taps at LDS byte address `addr`: rows I (top) and I + 1 (bot),
@@ -586,23 +599,30 @@ __device__ inline int4 read_rect(const int4 *tab, int c)
 // resource spanning exactly the view (raw 32-bit format), with the row
 // offset in an SGPR and the lane's column in the VGPR offset: no VALU work
 // per row, and any address outside the view would read zero.
-template <int PITCH>
+template <int PITCH, int CMAX, int NW>
 __device__ inline void stage_rect(const char *view, unsigned view_bytes, int pitch_u,
                                   int pitch_v, int u0, int v0, int h, int cols, unsigned *buf,
                                   int wave, int lane)
 {
-    if (lane >= cols) return;  // only the rectangle's columns are staged
     const __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(view), (short)0, (int)view_bytes,
                                           0x00020000);
-    const int col = u0 + lane;
-    const unsigned col4 = (unsigned)(col < 0 || col >= pitch_u ? pitch_u - 1 : col) * 4u;
+    // one load instruction per row and 64 columns; columns outside the view
+    // read its column 0 or its last column, both zero in every format
+    constexpr int kChunks = (CMAX + 63) / 64;
+    unsigned col4[kChunks];
+#pragma unroll
+    for (int c = 0; c < kChunks; ++c)
+        col4[c] = (unsigned)min(max(u0 + 64 * c + lane, 0), pitch_u - 1) * 4u;
     const unsigned row_bytes = (unsigned)pitch_u * 4u;
-    for (int r = wave; r < h; r += 4) {
+    for (int r = wave; r < h; r += NW) {
         const int srow = v0 + r < 0 ? pitch_v - 1 : min(v0 + r, pitch_v - 1);
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(
-            rsrc, (__attribute__((address_space(3))) void *)(buf + r * PITCH), 4, col4,
-            (unsigned)srow * row_bytes, 0, 0);
+#pragma unroll
+        for (int c = 0; c < kChunks; ++c)
+            if (64 * c + lane < cols)  // only the rectangle's columns are staged
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    rsrc, (__attribute__((address_space(3))) void *)(buf + r * PITCH + 64 * c),
+                    4, col4[c], (unsigned)srow * row_bytes, 0, 0);
     }
 }
 
@@ -720,8 +740,9 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
     const int pitch_u = J.pitch_u, pitch_v = J.pitch_v, N = J.n_views;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int i = bx * 16 + (wave & 1) * 8 + (lane & 7);
-    const int k = by * 16 + (wave >> 1) * 8 + (lane >> 3);
+    constexpr int TW = L::kTile / 8;  // waves along x
+    const int i = bx * L::kTile + (wave % TW) * 8 + (lane & 7);
+    const int k = by * L::kTile + (wave / TW) * 8 + (lane >> 3);
     const int j0 = bz * TY;
     // the run's rows relative to the volume centre, and the staged v centre
     const float oy = 0.5f - 0.5f * (float)J.Y;
@@ -743,7 +764,7 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
 
     // clear the row buffers (every LDS tap then reads a finite value), fill
     // the first rectangle table
-    for (int t = threadIdx.x; t < L::kBufDwords / 2; t += 256)
+    for (int t = threadIdx.x; t < L::kBufDwords / 2; t += L::kThreads)
         reinterpret_cast<uint4 *>(bufs)[t] = make_uint4(0u, 0u, 0u, 0u);
     if (threadIdx.x <= kRectChunk && (int)threadIdx.x < N)
         rects[threadIdx.x] = tile_rect(J, foot, foot_stride, vw(threadIdx.x), ja, jb, cv1, HMAX, CMAX, kBelow);
@@ -755,7 +776,8 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
 
     int4 cur = read_rect(rects, 0);
     if (rect_mode(cur) == RECT_LDS)
-        stage_rect<L::kPitch>(vbase + (size_t)vw(0) * view_bytes, view_bytes32, pitch_u, pitch_v,
+        stage_rect<L::kPitch, CMAX, L::kWaves>(vbase + (size_t)vw(0) * view_bytes, view_bytes32,
+                                               pitch_u, pitch_v,
                               cur.x, cur.y, cur.z, rect_cols(cur), bufs, wave, lane);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
@@ -769,7 +791,8 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
         if (n + 1 < N) {
             nxt = read_rect(rects, c + 1);
             if (rect_mode(nxt) == RECT_LDS)
-                stage_rect<L::kPitch>(vbase + (size_t)vw(n + 1) * view_bytes, view_bytes32,
+                stage_rect<L::kPitch, CMAX, L::kWaves>(vbase + (size_t)vw(n + 1) * view_bytes,
+                                                       view_bytes32,
                                       pitch_u, pitch_v, nxt.x, nxt.y, nxt.z, rect_cols(nxt),
                                       bufs + ((n + 1) & 1) * L::kBufDwords, wave, lane);
         }
@@ -841,7 +864,8 @@ constexpr int lds_min_blocks()
 }
 
 template <int DT, int TY>
-__global__ __launch_bounds__(256, lds_min_blocks<DT>()) void backproject_lds_kernel(BpJob J)
+__global__ __launch_bounds__(LdsLayout<DT>::kThreads, lds_min_blocks<DT>()) void backproject_lds_kernel(
+    BpJob J)
 {
     __shared__ __align__(16) unsigned lds[LdsLayout<DT>::kDwords];
     const int t = xcd_tile_order(blockIdx.x, gridDim.x);
@@ -854,7 +878,7 @@ __global__ __launch_bounds__(256, lds_min_blocks<DT>()) void backproject_lds_ker
 // Packed launch: the flat tile index selects the job (the last job whose
 // block_begin it reaches) and the tile inside it.
 template <int DT, int TY>
-__global__ __launch_bounds__(256, lds_min_blocks<DT>()) void backproject_lds_packed_kernel(
+__global__ __launch_bounds__(LdsLayout<DT>::kThreads, lds_min_blocks<DT>()) void backproject_lds_packed_kernel(
     BpBatch B)
 {
     __shared__ __align__(16) unsigned lds[LdsLayout<DT>::kDwords];
@@ -880,9 +904,17 @@ constexpr int kTY = CTR_LDS_TY;  // LDS kernel: voxels per thread along y
 // A 2048^2 panel is 2080 x 2051 staged texels (17 MB).
 constexpr size_t kMaxViewBytes = (size_t)1 << 31;
 
+// Column tile edge of the LDS kernel for a staged format.
+static int lds_tile(int staged_dtype)
+{
+    return staged_dtype == CTR_F32 ? LdsConfig<CTR_F32>::kTile : LdsConfig<CTR_F16>::kTile;
+}
+
+// `tile`: the column tile edge of the kernel that runs (lds_tile for the LDS
+// kernel, 16 for the direct-tap kernel).
 static BpJob make_job(const void *staged, const StagedLayout &L, int nu, int nv, GeomPtrs geom,
                       const float *view_scale, int n_views, float *vol, int X, int Y, int Z,
-                      float scale, int accumulate, int clip)
+                      float scale, int accumulate, int clip, int tile)
 {
     BpJob J;
     J.staged = staged;
@@ -902,8 +934,8 @@ static BpJob make_job(const void *staged, const StagedLayout &L, int nu, int nv,
     J.scale = scale;
     J.accumulate = accumulate;
     J.clip = clip;
-    J.gx = (X + 15) / 16;
-    J.gy = (Z + 15) / 16;
+    J.gx = (X + tile - 1) / tile;
+    J.gy = (Z + tile - 1) / tile;
     J.foot = nullptr;
     J.block_begin = 0;
     J.lds_pitch4 = 0;  // set per staged format at launch
@@ -918,34 +950,35 @@ static bool use_lds_kernel(int kernel_option, int Y)
     return kernel_option == CTR_KERNEL_LDS || (kernel_option == CTR_KERNEL_AUTO && Y >= 32);
 }
 
-// Fills the footprint table of job J at `dst` (gx * gy * n_views float4).
-static int build_footprints(hipStream_t s, BpJob &J, float4 *dst)
+// Fills the footprint table of job J at `dst` (gx * gy * n_views float4)
+// for tiles of `tile` x `tile` columns.
+static int build_footprints(hipStream_t s, BpJob &J, int tile, float4 *dst)
 {
     const long long total = (long long)J.gx * J.gy * J.n_views;
     footprint_kernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(J.geom64, J.n_views, J.gx,
-                                                                      J.gy, dst);
+                                                                      J.gy, tile, dst);
     CTR_CHECK_LAUNCH();
     J.foot = dst;
     return CTR_OK;
 }
 
-static size_t footprint_bytes(int X, int Z, int n_views)
+static size_t footprint_bytes(int X, int Z, int n_views, int tile)
 {
-    return sizeof(float4) * (size_t)((X + 15) / 16) * ((Z + 15) / 16) * n_views;
+    return sizeof(float4) * (size_t)((X + tile - 1) / tile) * ((Z + tile - 1) / tile) * n_views;
 }
 
 template <int DT>
 static void launch_lds(hipStream_t s, BpJob J, unsigned blocks)
 {
     J.lds_pitch4 = LdsConfig<DT>::kPitch * 4u;
-    backproject_lds_kernel<DT, kTY><<<blocks, 256, 0, s>>>(J);
+    backproject_lds_kernel<DT, kTY><<<blocks, LdsLayout<DT>::kThreads, 0, s>>>(J);
 }
 
 template <int DT>
 static void launch_lds_packed(hipStream_t s, BpBatch B, unsigned blocks)
 {
     for (int q = 0; q < B.count; ++q) B.job[q].lds_pitch4 = LdsConfig<DT>::kPitch * 4u;
-    backproject_lds_packed_kernel<DT, kTY><<<blocks, 256, 0, s>>>(B);
+    backproject_lds_packed_kernel<DT, kTY><<<blocks, LdsLayout<DT>::kThreads, 0, s>>>(B);
 }
 
 template <int DT>
@@ -1016,10 +1049,11 @@ static int backproject_with(hipStream_t s, int kernel_option, int geometry,
                                volume_width, rotation_count, rotation_offset, X, Y, Z, nu, nv, &g);
     if (rc) return rc;
     StagedLayout L = staged_layout(nu, nv, n_views, staged_dtype);
-    BpJob J = make_job(staged, L, nu, nv, g, view_scale, n_views, volume, X, Y, Z, scale,
-                       accumulate, clip_negative);
     const bool lds = use_lds_kernel(kernel_option, Y);
-    if (lds && (rc = build_footprints(s, J, (float4 *)foot_buf))) return rc;
+    const int tile = lds ? lds_tile(staged_dtype) : 16;
+    BpJob J = make_job(staged, L, nu, nv, g, view_scale, n_views, volume, X, Y, Z, scale,
+                       accumulate, clip_negative, tile);
+    if (lds && (rc = build_footprints(s, J, tile, (float4 *)foot_buf))) return rc;
     switch (staged_dtype) {
     case CTR_F32: return launch_bp<CTR_F32>(s, J, lds);
     case CTR_F16: return launch_bp<CTR_F16>(s, J, lds);
@@ -1081,7 +1115,8 @@ int ctr_backproject(void *stream, ctr_workspace *ws, int geometry, const void *s
     if ((rc = workspace_get(w, SLOT_GEOMETRY, kGeomBytesPerView * (size_t)n_views, &geom)))
         return rc;
     if (use_lds_kernel(kernel_option, Y) &&
-        (rc = workspace_get(w, SLOT_FOOTPRINT, footprint_bytes(X, Z, n_views), &foot)))
+        (rc = workspace_get(w, SLOT_FOOTPRINT,
+                            footprint_bytes(X, Z, n_views, lds_tile(staged_dtype)), &foot)))
         return rc;
     return backproject_with(as_stream(stream), kernel_option, geometry, staged,
                             staged_dtype, view_scale, nu, nv, n_views, source_distance,
@@ -1104,11 +1139,14 @@ int ctr_backproject_noalloc(void *stream, int geometry, const void *staged, int 
     if (rc) return rc;
     const int kernel_option = default_option(CTR_OPTION_BACKPROJECT_KERNEL);
     const size_t geom_bytes = (kGeomBytesPerView * (size_t)n_views + 255) & ~(size_t)255;
-    const size_t foot = use_lds_kernel(kernel_option, Y) ? footprint_bytes(X, Z, n_views) : 0;
+    const int tile = lds_tile(staged_dtype);
+    const size_t foot =
+        use_lds_kernel(kernel_option, Y) ? footprint_bytes(X, Z, n_views, tile) : 0;
     CTR_REQUIRE(geom_bytes + foot <= kNoAllocScratchBytes, CTR_ERR_OVER_BUDGET,
                 "ctr_backproject_noalloc: %d views over a %d x %d tile grid need %zu bytes of "
                 "scratch, more than its fixed %zu (use ctr_backproject)", n_views,
-                (X + 15) / 16, (Z + 15) / 16, geom_bytes + foot, kNoAllocScratchBytes);
+                (X + tile - 1) / tile, (Z + tile - 1) / tile, geom_bytes + foot,
+                kNoAllocScratchBytes);
     void *base = nullptr;
     if ((rc = scratch_address(&base))) return rc;
     return backproject_with(as_stream(stream), kernel_option, geometry, staged,
@@ -1130,7 +1168,7 @@ int ctr_backproject_packed(void *stream, ctr_workspace *ws, int staged_dtype,
     size_t total_views = 0, total_foot = 0;
     for (int q = 0; q < count; q++) {
         const ctr_backproject_job &j = jobs[q];
-        total_foot += footprint_bytes(j.X, j.Z, j.n_views);
+        total_foot += footprint_bytes(j.X, j.Z, j.n_views, lds_tile(staged_dtype));
         CTR_REQUIRE(j.staged && j.volume, CTR_ERR_NULL_POINTER,
                     "ctr_backproject_packed: job %d has a NULL buffer", q);
         CTR_REQUIRE(staged_dtype == CTR_F32 || j.view_scale, CTR_ERR_NULL_POINTER,
@@ -1167,13 +1205,14 @@ int ctr_backproject_packed(void *stream, ctr_workspace *ws, int staged_dtype,
                                     j.nv, &gp)))
             return rc;
         StagedLayout L = staged_layout(j.nu, j.nv, j.n_views, staged_dtype);
+        const int tile = lds_tile(staged_dtype);
         B.job[q] = make_job(j.staged, L, j.nu, j.nv, gp, j.view_scale, j.n_views, j.volume, j.X,
-                            j.Y, j.Z, j.scale, j.accumulate, j.clip_negative);
-        if ((rc = build_footprints(s, B.job[q], (float4 *)fp))) return rc;
+                            j.Y, j.Z, j.scale, j.accumulate, j.clip_negative, tile);
+        if ((rc = build_footprints(s, B.job[q], tile, (float4 *)fp))) return rc;
         B.job[q].block_begin = (int)blocks;
         blocks += (long long)B.job[q].gx * B.job[q].gy * ((j.Y + kTY - 1) / kTY);
         g += kGeomBytesPerView * (size_t)j.n_views;
-        fp += footprint_bytes(j.X, j.Z, j.n_views);
+        fp += footprint_bytes(j.X, j.Z, j.n_views, tile);
     }
     CTR_REQUIRE(blocks < (1LL << 31), CTR_ERR_INVALID_ARGUMENT, "packed launch too large");
     switch (staged_dtype) {

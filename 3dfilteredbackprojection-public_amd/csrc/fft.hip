@@ -1166,8 +1166,7 @@ __device__ inline unsigned bf16_step_word(float t, unsigned lo)
 // of the step alone.  fp16: d rounded to fp16.  bf16: the upper half is
 // chosen so that the whole word, read as FP32, is nearest to d
 // (bf16_step_word).  Row 0 (the zero border) holds (0, q[1][c]): the first
-// data row is its step.  One workgroup per staged row, four texels per
-// thread.
+// data row is its step.
 __device__ inline unsigned pack_vstep(float v, float w, int bf16)
 {
     if (bf16) {
@@ -1179,31 +1178,42 @@ __device__ inline unsigned pack_vstep(float v, float w, int bf16)
            ((unsigned)f32_to_f16_bits(w - (float)hv) << 16);
 }
 
+constexpr int kQuantizeBand = 16;  // staged rows per workgroup of quantize_vstep_kernel
+
+// One workgroup per band of kQuantizeBand staged rows of one view (blockIdx.x
+// = view * bands + band), four texels per thread: every row is read once
+// and kept in registers as the previous row's "next" (the band reads one
+// row past its end).
 __global__ __launch_bounds__(256) void quantize_vstep_kernel(const float *st, int pitch_u,
                                                              int pitch_v, float *view_scale,
                                                              unsigned *out, int bf16)
 {
-    const unsigned row = blockIdx.x;  // staged row over all views
-    const unsigned n = row / (unsigned)pitch_v;
-    const bool last = row % (unsigned)pitch_v == (unsigned)pitch_v - 1;
+    const int bands = (pitch_v + kQuantizeBand - 1) / kQuantizeBand;
+    const int n = blockIdx.x / bands;
+    const int r0 = (blockIdx.x % bands) * kQuantizeBand;
+    const int r1 = min(r0 + kQuantizeBand, pitch_v);
     float vs = view_scale[n];
     if (!(vs > 0.0f)) {
         vs = 1.0f;
-        if (threadIdx.x == 0 && row % (unsigned)pitch_v == 0) view_scale[n] = 1.0f;
+        if (threadIdx.x == 0 && r0 == 0) view_scale[n] = 1.0f;
     }
     const float inv = 1.0f / vs;
-    const float *src = st + (size_t)row * pitch_u;
-    unsigned *dst = out + (size_t)row * pitch_u;
+    const float *src = st + (size_t)n * pitch_v * pitch_u;
+    unsigned *dst = out + (size_t)n * pitch_v * pitch_u;
     for (int c = 4 * threadIdx.x; c < pitch_u; c += 4 * blockDim.x) {
-        const float4 a = *reinterpret_cast<const float4 *>(src + c);
-        const float4 b = last ? make_float4(0.0f, 0.0f, 0.0f, 0.0f)
-                              : *reinterpret_cast<const float4 *>(src + pitch_u + c);
-        uint4 o;
-        o.x = pack_vstep(a.x * inv, b.x * inv, bf16);
-        o.y = pack_vstep(a.y * inv, b.y * inv, bf16);
-        o.z = pack_vstep(a.z * inv, b.z * inv, bf16);
-        o.w = pack_vstep(a.w * inv, b.w * inv, bf16);
-        *reinterpret_cast<uint4 *>(dst + c) = o;
+        float4 a = *reinterpret_cast<const float4 *>(src + (size_t)r0 * pitch_u + c);
+        for (int r = r0; r < r1; ++r) {
+            const float4 b = r + 1 < pitch_v
+                                 ? *reinterpret_cast<const float4 *>(src + (size_t)(r + 1) * pitch_u + c)
+                                 : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            uint4 o;
+            o.x = pack_vstep(a.x * inv, b.x * inv, bf16);
+            o.y = pack_vstep(a.y * inv, b.y * inv, bf16);
+            o.z = pack_vstep(a.z * inv, b.z * inv, bf16);
+            o.w = pack_vstep(a.w * inv, b.w * inv, bf16);
+            *reinterpret_cast<uint4 *>(dst + (size_t)r * pitch_u + c) = o;
+            a = b;
+        }
     }
 }
 
@@ -1225,10 +1235,12 @@ static int finish_staged(hipStream_t s, const float *st32, int nu, int nv, int n
         view_max_kernel<<<n_views, 256, 0, s>>>(st32, L.view, view_scale);
         CTR_CHECK_LAUNCH();
     }
-    const long long rows = (long long)L.pitch_v * n_views;
-    CTR_REQUIRE(rows < (1LL << 31), CTR_ERR_INVALID_ARGUMENT, "too many staged rows");
-    quantize_vstep_kernel<<<(unsigned)rows, 256, 0, s>>>(st32, L.pitch_u, L.pitch_v, view_scale,
-                                                         (unsigned *)staged, dtype == CTR_BF16);
+    const long long blocks =
+        (long long)((L.pitch_v + kQuantizeBand - 1) / kQuantizeBand) * n_views;
+    CTR_REQUIRE(blocks < (1LL << 31), CTR_ERR_INVALID_ARGUMENT, "too many staged rows");
+    quantize_vstep_kernel<<<(unsigned)blocks, 256, 0, s>>>(st32, L.pitch_u, L.pitch_v,
+                                                           view_scale, (unsigned *)staged,
+                                                           dtype == CTR_BF16);
     CTR_CHECK_LAUNCH();
     return CTR_OK;
 }
