@@ -89,10 +89,10 @@ def test_c3_storage_forms_vs_oracle(ctr, orc, dev):
     (per-view scale, FP32 interpolation and accumulation, FP64 geometry).
     Each reconstruction is checked against the FP64 oracle backprojection of
     the FP32 filtered projections on the central 64^3 crop: FP32 max-rel
-    <= 1e-4, 16-bit rel RMSE <= 1e-3.  (Measured about 8e-7 for FP32; 6e-6
-    and 5e-5 for FP16 and BF16 stored as (q[u], q[u + 1]) pairs.  Without
-    the body the crop is empty space, where the same absolute error over a
-    reference 160 times smaller reads as 1.5e-3 / 6e-3.)"""
+    <= 1e-4, 16-bit rel RMSE <= 1e-3.  (Measured about 8e-7 for FP32; 4.7e-6
+    and 3.8e-5 for FP16 and BF16.  Without the body the crop is empty
+    space, where the same absolute error over a reference 160 times smaller
+    reads as 1e-3 and more.)"""
     import torch
     g = ctr.config_geometry("fdk_1440")
     ell = ctr.random_ellipsoids(3, 60, (g.X, g.Y, g.Z), g.voxel, (5, 240))
