@@ -129,7 +129,7 @@ StagedLayout staged_layout(int nu, int nv, int n_views, int dtype)
     L.pitch_u = (nu + 3 + 31) & ~31;
     L.pitch_v = nv + 3;             // one zero row before, two after
     L.view = (size_t)L.pitch_u * (size_t)L.pitch_v;
-    size_t texel = 4;               // fp32 texel, or a 16-bit pair
+    size_t texel = 4;               // fp32 value, or a 16-bit value and its step
     (void)dtype;
     L.bytes = L.view * (size_t)n_views * texel;
     return L;

@@ -24,6 +24,11 @@ Design:
 * Small requests share *packed slots*: up to `pack` small reconstructions
   are filtered and then backprojected by ONE ctr_backproject_packed launch.
   There is one graph per fill count.
+* Large reconstructions are backprojected in consecutive *view chunks*, one
+  accumulating launch per chunk (the volume is read and written once per
+  chunk).  A launch fills the whole GPU until it drains, so work of other
+  slots waits for the launch in flight; with chunks it waits for one chunk
+  instead of the whole reconstruction.
 * Admission runs against an HBM budget.  A request is admitted as soon as a
   free slot of its class exists or a new one fits the budget; idle slots of
   other classes are evicted to make room.  Requests that arrive while others
@@ -36,6 +41,7 @@ Design:
   (tests/test_gpu_serving.py checks alone-vs-mixed equality).
 """
 import collections
+import dataclasses
 import math
 import statistics
 import time
@@ -88,13 +94,30 @@ def make_projections(geom, seed, out=None, device=None, stream=None):
                               stream=stream)
 
 
+def view_chunks(geom, chunks):
+    """The views of `geom` split into `chunks` consecutive ranges: a list of
+    (first view, geometry of the range).  Views i0 .. i0 + n - 1 of a scan of
+    N views are themselves a scan of n views with rotation count rot n / N
+    and offset offset + 2 pi rot i0 / N, since view i is at the angle
+    2 pi rot i / N + offset (FBP3D.cpp:536-540)."""
+    N = geom.n_views
+    chunks = max(1, min(int(chunks), N))
+    bounds = [N * c // chunks for c in range(chunks + 1)]
+    return [(i0, dataclasses.replace(
+                geom, n_views=i1 - i0, rotation_count=geom.rotation_count * (i1 - i0) / N,
+                rotation_offset=geom.rotation_offset + 2 * math.pi * geom.rotation_count * i0 / N))
+            for i0, i1 in zip(bounds, bounds[1:])]
+
+
 class Slot:
     """Buffers, stream and captured graph for `count` reconstructions of one
-    class (count > 1: a packed slot)."""
+    class (count > 1: a packed slot; a single one is backprojected in
+    `chunks` view chunks)."""
 
-    def __init__(self, cls, geom, dtype, count=1, use_graph=True):
+    def __init__(self, cls, geom, dtype, count=1, use_graph=True, chunks=1):
         import torch
         self.cls, self.geom, self.dtype, self.count = cls, geom, dtype, count
+        self.parts = view_chunks(geom, chunks) if count == 1 else [(0, geom)]
         self.fft = fft_length_for(geom.nu)
         self.scale = geom.fbp_scale(self.fft)
         dev = torch.device("cuda")
@@ -109,6 +132,7 @@ class Slot:
         self.vs = [torch.empty(g.n_views, dtype=torch.float32, device=dev) for _ in range(count)]
         self.vol = [torch.empty((g.Z, g.Y, g.X), dtype=torch.float32, device=dev)
                     for _ in range(count)]
+        self.view_bytes = sbytes // g.n_views
         self.sums = torch.zeros(count, dtype=torch.float64, device=dev)
         self.bytes = count * (4 * g.n_views * g.nv * g.nu + sbytes + 4 * g.n_views +
                               4 * g.X * g.Y * g.Z) + (s32 if dtype != F32 else 0)
@@ -136,8 +160,12 @@ class Slot:
             filter_projections(self.proj[q], g, RAM_LAK, self.fft, self.dtype, fdk=True,
                                ws=self.ws, staged=self.staged[q], view_scale=self.vs[q])
         if self.count == 1:
-            backproject(self.staged[0], g, self.dtype, view_scale=self.vs[0], scale=self.scale,
-                        volume=vols[0], clip=True, ws=self.ws)
+            last = len(self.parts) - 1
+            for c, (i0, gc) in enumerate(self.parts):
+                i1 = i0 + gc.n_views
+                backproject(self.staged[0][i0 * self.view_bytes:i1 * self.view_bytes], gc,
+                            self.dtype, view_scale=self.vs[0][i0:i1], scale=self.scale,
+                            volume=vols[0], accumulate=c > 0, clip=c == last, ws=self.ws)
         else:
             backproject_packed([(self.staged[q], g, self.vs[q], self.scale, vols[q], True)
                                 for q in range(fill)], self.dtype, ws=self.ws)
@@ -167,13 +195,14 @@ class Server:
     """Admission, slot management and the serving loop."""
 
     def __init__(self, budget_bytes=200e9, dtype=F16, pack=4, max_slots=None,
-                 use_graphs=True, keep_results=False, log=None, geometries=None):
+                 use_graphs=True, keep_results=False, log=None, geometries=None, chunks=None):
         import torch
         self.device = torch.device("cuda")
         self.budget = float(budget_bytes)
         self.dtype = dtype
         self.pack = max(1, min(pack, MAX_PACKED))
         self.max_slots = max_slots or {"small": 2, "medium": 6, "large": 4}
+        self.chunks = {"small": 1, "medium": 1, "large": 8, **(chunks or {})}
         self.use_graphs = use_graphs
         self.keep_results = keep_results
         self.geoms = geometries or {c: config_geometry(w) for c, w in CLASSES.items()}
@@ -211,12 +240,16 @@ class Server:
                 self._drop(s)
             if self.used + need > self.budget:
                 return None
-        slot = Slot(cls, self.geoms[cls], self.dtype, self._slot_count(cls), self.use_graphs)
+        slot = self._new_slot(cls)
         self.slots.append(slot)
         self.used += slot.bytes
         self.log(f"slot {cls} x{slot.count}: {slot.bytes / 1e9:.2f} GB, in use "
                  f"{self.used / 1e9:.1f} / {self.budget / 1e9:.0f} GB")
         return slot
+
+    def _new_slot(self, cls):
+        return Slot(cls, self.geoms[cls], self.dtype, self._slot_count(cls), self.use_graphs,
+                    self.chunks[cls])
 
     def _drop(self, slot):
         self.slots.remove(slot)
@@ -232,7 +265,7 @@ class Server:
             for _ in range(self.max_slots[c]):
                 if self.used + self._estimate(c) > self.budget:
                     break
-                s = Slot(c, self.geoms[c], self.dtype, self._slot_count(c), self.use_graphs)
+                s = self._new_slot(c)
                 self.slots.append(s)
                 self.used += s.bytes
                 made.append(s)

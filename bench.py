@@ -164,17 +164,18 @@ def cpu_baseline(g, proj_cpu, scale, threads, full=True, budget_s=6.0):
             "kind": "port", "seconds": round(total, 2), "sample": sample}
 
 
-def run_serving(ctr, requests, rate, budget_gb, pack, max_slots):
+def run_serving(ctr, requests, rate, budget_gb, pack, max_slots, chunks):
     """Poisson stream of FDK requests over three sizes (256^3/360/384^2,
     512^3/720/768^2, 1024^3/1200/1024^2), each with its own seeded phantom,
     FP16 filtered projections, FP32 volumes: p50/p99 latency and aggregate
     GUPS."""
     import torch
     from ctrecon import serving
-    small, medium, large = (int(x) for x in max_slots.split(","))
-    srv = serving.Server(budget_bytes=budget_gb * 1e9, dtype=ctr.F16, pack=pack,
-                         max_slots={"small": small, "medium": medium, "large": large}, log=log)
     classes = ["small", "medium", "large"]
+    slots = dict(zip(classes, (int(x) for x in max_slots.split(","))))
+    view_chunks = dict(zip(classes, (int(x) for x in chunks.split(","))))
+    srv = serving.Server(budget_bytes=budget_gb * 1e9, dtype=ctr.F16, pack=pack,
+                         max_slots=slots, chunks=view_chunks, log=log)
     srv.warm(classes)
     torch.cuda.synchronize()
     reqs = serving.poisson_arrivals(requests, rate, seed=7)
@@ -196,6 +197,7 @@ def run_serving(ctr, requests, rate, budget_gb, pack, max_slots):
         "slots": {c: sum(1 for x in srv.slots if x.cls == c) for c in classes},
         "slot_gb": round(srv.used / 1e9, 2),
         "pack": pack,
+        "view_chunks": view_chunks,
         "config": "sizes uniform over 256^3/360 views/384^2, 512^3/720/768^2, "
                   "1024^3/1200/1024^2; FP16 store, FP32 volume, FFT 2 nu",
         "data": "each request its own seeded ellipsoid phantom; its exact projections are "
@@ -219,6 +221,10 @@ def main():
     # several in flight delay every smaller request behind them
     ap.add_argument("--max-slots", default="2,6,1",
                     help="serving slots per class: small,medium,large")
+    # a 1024^3 reconstruction in 8 accumulating launches of 150 views: other
+    # slots' work waits for one ~40 ms chunk instead of the whole ~0.3 s
+    ap.add_argument("--chunks", default="1,1,8",
+                    help="serving view chunks per reconstruction: small,medium,large")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: there is no multi-GPU path "
@@ -243,7 +249,7 @@ def main():
     ctr.require_device()
     if args.mode == "serving":
         print(json.dumps(run_serving(ctr, args.requests, args.rate, args.budget_gb, args.pack,
-                                     args.max_slots)), flush=True)
+                                     args.max_slots, args.chunks)), flush=True)
         return
 
     dev = torch.device("cuda:0")
@@ -345,7 +351,7 @@ def main():
     serving_line = None
     if not args.no_serving:
         serving_line = run_serving(ctr, args.requests, args.rate, args.budget_gb, args.pack,
-                                   args.max_slots)
+                                   args.max_slots, args.chunks)
         log(f"serving p50 {serving_line['latency_ms']['p50']} ms, "
             f"p99 {serving_line['latency_ms']['p99']} ms")
 

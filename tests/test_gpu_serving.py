@@ -8,6 +8,8 @@
   and at the serving workload's real sizes (1024^3 from 1200 views of 1024^2
   mixed with packed 256^3 and a 512^3), where one served volume is also
   checked against the FP64 oracle on a crop.
+* A reconstruction served in view chunks equals the single launch up to
+  FP32 rounding.
 * Each request's projections are its own seeded phantom's exact line
   integrals (ctr_project_ellipsoids, checked against the oracle's analytic
   projector).
@@ -78,6 +80,28 @@ def test_alone_equals_mixed(ctr, dev):
         assert alone.checksum == r.checksum
     summary = serving.summarize(mix, makespan, geoms)
     assert summary["requests"] == len(mix) and summary["aggregate_gups"] > 0
+
+
+@pytest.mark.parametrize("chunks", [2, 5])
+def test_view_chunks_match_one_launch(ctr, dev, chunks):
+    """A reconstruction backprojected in view chunks (one accumulating launch
+    per chunk, clipped after the last, replayed from the slot's graph) equals
+    the single launch up to the FP32 rounding of its partial sums."""
+    import torch
+    from ctrecon import serving
+    g = _geoms(ctr)["large"]
+    one = serving.Slot("large", g, ctr.F16)
+    many = serving.Slot("large", g, ctr.F16, chunks=chunks)
+    assert len(many.parts) == chunks
+    out = []
+    for slot in (one, many):
+        for seed in (3, 4):
+            slot.launch([serving.Request(seed, "large", seed, 0.0)], keep_results=True)
+            torch.cuda.synchronize()
+            out.append(slot.results[0].cpu().numpy())
+    for a, b in zip(out[:2], out[2:]):
+        assert np.abs(a).max() > 0
+        assert rel_max(b, a) < 2e-6, rel_max(b, a)
 
 
 def test_server_respects_budget(ctr, dev):

@@ -55,3 +55,24 @@ def test_summary_percentiles(ctr, serving):
     assert s["aggregate_gups"] == pytest.approx(updates / 2.0 / 1e9)
     assert s["per_class"]["small"]["n"] == 100
     assert math.isclose(s["mean_ms"], 100.5)
+
+
+@pytest.mark.parametrize("chunks", [1, 3, 7, 120])
+def test_view_chunks_cover_the_scan(ctr, serving, chunks):
+    """Each chunk is the scan restricted to consecutive views: together they
+    hold every view once, and chunk view m of a chunk starting at i0 has the
+    angle of scan view i0 + m (angle 2 pi rot i / N + offset)."""
+    g = ctr.Geometry(ctr.CONE, nu=64, nv=48, n_views=120, X=40, Y=36, Z=40,
+                     detector_width=120.0, volume_width=40.0, source_distance=150.0,
+                     detector_distance=100.0, rotation_count=0.75, rotation_offset=0.3)
+    parts = serving.view_chunks(g, chunks)
+    assert len(parts) == chunks
+    angle = lambda geo, i: 2 * math.pi * geo.rotation_count * i / geo.n_views + geo.rotation_offset
+    seen = []
+    for i0, gc in parts:
+        assert gc.n_views >= 1 and (gc.nu, gc.nv, gc.X, gc.kind) == (g.nu, g.nv, g.X, g.kind)
+        for m in range(gc.n_views):
+            assert angle(gc, m) == pytest.approx(angle(g, i0 + m), abs=1e-12)
+        seen += range(i0, i0 + gc.n_views)
+    assert seen == list(range(g.n_views))
+    assert serving.view_chunks(g, 1)[0][1] == g
