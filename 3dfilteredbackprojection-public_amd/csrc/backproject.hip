@@ -839,7 +839,28 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
 }
 
 // Tiles reach the XCDs through xcd_tile_order: neighbouring tiles, which read
-// overlapping detector rows, share one XCD's L2.
+// overlapping detector rows, share one XCD's L2.  Inside one y-run layer the
+// flat index walks bands of kTileBand tile rows along z, column by column, so
+// that tiles dispatched together form a compact patch of columns.  Measured
+// (FP16 / FP32 kernel ms, 512^3 x 720): rows along x (band 1) 25.1 / 33.2,
+// band 2 24.2 / 32.3, bands 3-6 23.3-23.5 / 31.5-31.9, band 8 23.8 / 31.9,
+// band 16 24.5 / 32.7; at 1024^3 x 1200 band 4 is 4% faster than band 1.
+#ifndef CTR_TILE_BAND
+#define CTR_TILE_BAND 4
+#endif
+constexpr int kTileBand = CTR_TILE_BAND;
+
+__device__ inline void lds_tile_coords(int t, const BpJob &J, int &bx, int &by, int &bz)
+{
+    const int layer = J.gx * J.gy;
+    bz = t / layer;
+    const int r = t - bz * layer;
+    const int band = r / (J.gx * kTileBand);
+    const int h = min(kTileBand, J.gy - band * kTileBand);
+    const int rb = r - band * J.gx * kTileBand;
+    bx = rb / h;
+    by = band * kTileBand + (rb - bx * h);
+}
 
 // Waves per SIMD the compiler must leave registers for (the second launch
 // bound; one wave of each workgroup per SIMD, so also workgroups per CU), fp32
@@ -868,10 +889,8 @@ __global__ __launch_bounds__(LdsLayout<DT>::kThreads, lds_min_blocks<DT>()) void
     BpJob J)
 {
     __shared__ __align__(16) unsigned lds[LdsLayout<DT>::kDwords];
-    const int t = xcd_tile_order(blockIdx.x, gridDim.x);
-    const int bx = t % J.gx;
-    const int by = (t / J.gx) % J.gy;
-    const int bz = t / (J.gx * J.gy);
+    int bx, by, bz;
+    lds_tile_coords(xcd_tile_order(blockIdx.x, gridDim.x), J, bx, by, bz);
     backproject_lds_tile<DT, TY>(J, bx, by, bz, lds);
 }
 
@@ -888,10 +907,8 @@ __global__ __launch_bounds__(LdsLayout<DT>::kThreads, lds_min_blocks<DT>()) void
     for (int q = 1; q < B.count; ++q)
         if (b >= B.job[q].block_begin) j = q;
     const BpJob &J = B.job[j];
-    const int local = b - J.block_begin;
-    const int bx = local % J.gx;
-    const int by = (local / J.gx) % J.gy;
-    const int bz = local / (J.gx * J.gy);
+    int bx, by, bz;
+    lds_tile_coords(b - J.block_begin, J, bx, by, bz);
     backproject_lds_tile<DT, TY>(J, bx, by, bz, lds);
 }
 

@@ -20,7 +20,9 @@ rate of MI355X's published 157.3 TFLOP/s FP32 vector peak: 157.3e12 / 2
 FLOP per FMA / 64 lanes = 1229 G wave-instructions/s.  That published rate
 counts packed FP32 (two FMAs per lane per instruction), which this build
 does not use; beside it the line gives the rate a probe of the kernel's own
-instruction mix reaches on the box (profiles/).  The HBM side is reported
+instruction mix reaches (probes/valu_issue, run on the same GPU after the
+timing; its kernels are short and run at a higher clock than the
+backprojector's, so the kernel can come close to it or pass it).  The HBM side is reported
 too: bytes measured per launch (FETCH_SIZE + WRITE_SIZE, scaled by a copy of
 known size, profiles/backproject_fdk512_traffic.md) over the same duration,
 against 8 TB/s, and the tap bytes the LDS serves.
@@ -72,6 +74,25 @@ def profile_value(name, key):
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def probe_mix_ceiling():
+    """The issue rate (wave64 VALU instructions per ns per SIMD, 5 waves per
+    SIMD) that the probe probes/valu_issue reaches with the FP16 loop's
+    instruction mix, run now as a child process on this GPU; None when the
+    probe is not built or fails."""
+    import subprocess
+    exe = os.path.join(ROOT, "3dfilteredbackprojection-public_amd", "probes", "valu_issue")
+    if not os.path.exists(exe):
+        return None
+    try:
+        out = subprocess.run([exe], capture_output=True, text=True, timeout=60).stdout
+    except (OSError, subprocess.SubprocessError):
+        return None
+    for line in out.splitlines():
+        if line.startswith("loop mix fp16"):
+            return float(line.split()[4])
+    return None
 
 
 def make_inputs(ctr, g, dev, seed=20240602):
@@ -311,9 +332,13 @@ def main():
     v16, achieved16 = valu("fp16", bp_ms)
     v32, achieved32 = valu("fp32", bp_ms32)
     # the issue rate a probe of the fp16 loop's own instruction mix reaches
-    # (wave64 VALU instructions per ns per SIMD, independent chains)
-    mix_ceiling = profile_value("backproject_fdk512_sq_counters.md",
-                                "fp16_mix_issue_ceiling_per_ns")
+    # (wave64 VALU instructions per ns per SIMD, independent chains): on this
+    # GPU now, else the figure recorded with the counters
+    mix_ceiling, mix_source = probe_mix_ceiling(), "probe run on this GPU"
+    if mix_ceiling is None:
+        mix_ceiling = profile_value("backproject_fdk512_sq_counters.md",
+                                    "fp16_mix_issue_ceiling_per_ns")
+        mix_source = "profiles/backproject_fdk512_sq_counters.md"
     traffic16 = profile_value("backproject_fdk512_traffic.md", "fp16_bytes_per_launch")
     traffic32 = profile_value("backproject_fdk512_traffic.md", "fp32_bytes_per_launch")
     tap_bytes16 = (4 * 2 + 8.0 / g.n_views) * g.updates
@@ -399,6 +424,8 @@ def main():
             "unit": "G wave64 VALU instructions/s",
             "frac": round(achieved16 / VALU_PEAK_GINSTR, 4) if achieved16 else None,
             "issue_per_ns_per_simd": (round(achieved16 / 1024, 3) if achieved16 else None),
+            "mix_ceiling_per_ns_per_simd": mix_ceiling,
+            "mix_ceiling_source": mix_source if mix_ceiling else None,
             "frac_of_mix_ceiling": (round(achieved16 / 1024 / mix_ceiling, 4)
                                     if achieved16 and mix_ceiling else None),
             "valu_per_update": v16,
