@@ -838,17 +838,45 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
     store_run<TY>(acc, J.vol, i, j0, k, J.X, J.Y, J.Z, J.accumulate, J.clip);
 }
 
-// Tiles reach the XCDs through xcd_tile_order: neighbouring tiles, which read
-// overlapping detector rows, share one XCD's L2.  Inside one y-run layer the
-// flat index walks bands of kTileBand tile rows along z, column by column, so
-// that tiles dispatched together form a compact patch of columns.  Measured
-// (FP16 / FP32 kernel ms, 512^3 x 720): rows along x (band 1) 25.1 / 33.2,
-// band 2 24.2 / 32.3, bands 3-6 23.3-23.5 / 31.5-31.9, band 8 23.8 / 31.9,
-// band 16 24.5 / 32.7; at 1024^3 x 1200 band 4 is 4% faster than band 1.
+// Tile order.  Workgroups reach the 8 XCDs round-robin (block b on XCD
+// b mod 8).  bp_tile_order deals the tiles to the XCDs in chunks of
+// kXcdChunk consecutive tiles, chunk c to XCD c mod 8, so that the XCDs work
+// on neighbouring chunks of the same y-run layer at the same time and read
+// the same detector rows (with one contiguous range of tiles per XCD, each
+// XCD read rows of its own).  Inside a layer, lds_tile_coords walks bands of kTileBand tile
+// rows along z, column by column, so that tiles dispatched together form a
+// compact patch of columns whose rows they share in L2.
+//
+// Measured (FP16 / FP32 kernel ms):
+//   512^3 x 720: rows along x, one contiguous range per XCD 25.1 / 33.2;
+//     bands of 4: 23.3-23.6 / 31.7-32.3 (bands of 3-6 alike, 2, 8, 16
+//     slower; 4 x 4 squares, serpentine or x bands no better); with chunks
+//     of 128 tiles dealt round-robin 22.7-23.0 / 29.4-30.0 (chunks of 32, 64,
+//     256 and 512 between the two).  Bytes fetched from beyond L2 per FP16
+//     launch: 70.3, 43.8 and 31.2 GB.
+//   1024^3 x 1200: 287.6 / 362.1, 274.4-279.7 / 349.4-354.1, 275.7-276.2 /
+//     350.1-355.0.
+//   1440^3 x 1440: bands of 4 in ranges 965 / 1159; chunks of 128 828 / 1014.
 #ifndef CTR_TILE_BAND
 #define CTR_TILE_BAND 4
 #endif
 constexpr int kTileBand = CTR_TILE_BAND;
+#ifndef CTR_XCD_CHUNK
+#define CTR_XCD_CHUNK 128
+#endif
+constexpr int kXcdChunk = CTR_XCD_CHUNK;
+
+// A bijection of [0, nblocks): XCD x's k-th block takes tile
+// ((k / C) * 8 + x) * C + k mod C; the tail past the last whole round of
+// 8 chunks keeps its own index.
+__device__ inline int bp_tile_order(int b, int nblocks)
+{
+    constexpr int C = kXcdChunk;
+    const int full = nblocks / (8 * C) * (8 * C);
+    if (b >= full) return b;
+    const int x = b & 7, k = b >> 3;
+    return ((k / C) * 8 + x) * C + k % C;
+}
 
 __device__ inline void lds_tile_coords(int t, const BpJob &J, int &bx, int &by, int &bz)
 {
@@ -890,7 +918,7 @@ __global__ __launch_bounds__(LdsLayout<DT>::kThreads, lds_min_blocks<DT>()) void
 {
     __shared__ __align__(16) unsigned lds[LdsLayout<DT>::kDwords];
     int bx, by, bz;
-    lds_tile_coords(xcd_tile_order(blockIdx.x, gridDim.x), J, bx, by, bz);
+    lds_tile_coords(bp_tile_order(blockIdx.x, gridDim.x), J, bx, by, bz);
     backproject_lds_tile<DT, TY>(J, bx, by, bz, lds);
 }
 
@@ -901,7 +929,7 @@ __global__ __launch_bounds__(LdsLayout<DT>::kThreads, lds_min_blocks<DT>()) void
     BpBatch B)
 {
     __shared__ __align__(16) unsigned lds[LdsLayout<DT>::kDwords];
-    const int b = xcd_tile_order(blockIdx.x, gridDim.x);
+    const int b = bp_tile_order(blockIdx.x, gridDim.x);
     int j = 0;
 #pragma unroll 1
     for (int q = 1; q < B.count; ++q)

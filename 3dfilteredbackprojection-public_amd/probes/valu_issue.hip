@@ -19,7 +19,9 @@
 
 namespace {
 
-constexpr int kIter = 1024;
+// About 1 ms per launch, so that the launch and the waves' ramp up and down
+// are about 1% of the time.
+constexpr int kIter = 16384;
 
 #define CHECK(call)                                                                     \
     do {                                                                                \
