@@ -1183,10 +1183,12 @@ constexpr int kQuantizeBand = 16;  // staged rows per workgroup of quantize_vste
 // One workgroup per band of kQuantizeBand staged rows of one view (blockIdx.x
 // = view * bands + band), four texels per thread: every row is read once
 // and kept in registers as the previous row's "next" (the band reads one
-// row past its end).
-__global__ __launch_bounds__(256) void quantize_vstep_kernel(const float *st, int pitch_u,
-                                                             int pitch_v, float *view_scale,
-                                                             unsigned *out, int bf16)
+// row past its end).  Only the detector texels (rows 1..nv, columns 1..nu)
+// of `st` are read; the border is taken as zero, whatever `st` holds there.
+__global__ __launch_bounds__(256) void quantize_vstep_kernel(const float *st, int nu, int nv,
+                                                             int pitch_u, int pitch_v,
+                                                             float *view_scale, unsigned *out,
+                                                             int bf16)
 {
     const int bands = (pitch_v + kQuantizeBand - 1) / kQuantizeBand;
     const int n = blockIdx.x / bands;
@@ -1201,11 +1203,20 @@ __global__ __launch_bounds__(256) void quantize_vstep_kernel(const float *st, in
     const float *src = st + (size_t)n * pitch_v * pitch_u;
     unsigned *dst = out + (size_t)n * pitch_v * pitch_u;
     for (int c = 4 * threadIdx.x; c < pitch_u; c += 4 * blockDim.x) {
-        float4 a = *reinterpret_cast<const float4 *>(src + (size_t)r0 * pitch_u + c);
+        // texel c + i is a detector column when 1 <= c + i <= nu (selected,
+        // not multiplied by a 0/1 mask, so that the border is +0 whatever
+        // its sign or value in `st`)
+        const bool m0 = c >= 1 && c <= nu, m1 = c + 1 <= nu, m2 = c + 2 <= nu, m3 = c + 3 <= nu;
+        const bool any = c <= nu;
+        auto row = [&](int r) {
+            if (!any || r < 1 || r > nv) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const float4 v = *reinterpret_cast<const float4 *>(src + (size_t)r * pitch_u + c);
+            return make_float4(m0 ? v.x : 0.0f, m1 ? v.y : 0.0f, m2 ? v.z : 0.0f,
+                               m3 ? v.w : 0.0f);
+        };
+        float4 a = row(r0);
         for (int r = r0; r < r1; ++r) {
-            const float4 b = r + 1 < pitch_v
-                                 ? *reinterpret_cast<const float4 *>(src + (size_t)(r + 1) * pitch_u + c)
-                                 : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const float4 b = row(r + 1);
             uint4 o;
             o.x = pack_vstep(a.x * inv, b.x * inv, bf16);
             o.y = pack_vstep(a.y * inv, b.y * inv, bf16);
@@ -1222,8 +1233,10 @@ static inline unsigned grid1d(long long total, int block)
     return (unsigned)((total + block - 1) / block);
 }
 
-// fp32 staged (borders zero) -> caller's staged buffer in `dtype`.  With
-// `have_max` the per-view maxima are already in view_scale (from the filter).
+// fp32 staged -> caller's staged buffer in `dtype`.  With `have_max` the
+// per-view maxima are already in view_scale (from the filter, which writes the
+// detector texels of `st32` only); without, `st32` is whole, border zero
+// included (stage_plain_kernel), and view_max_kernel reads all of it.
 static int finish_staged(hipStream_t s, const float *st32, int nu, int nv, int n_views,
                          void *staged, int dtype, float *view_scale, bool have_max)
 {
@@ -1238,7 +1251,7 @@ static int finish_staged(hipStream_t s, const float *st32, int nu, int nv, int n
     const long long blocks =
         (long long)((L.pitch_v + kQuantizeBand - 1) / kQuantizeBand) * n_views;
     CTR_REQUIRE(blocks < (1LL << 31), CTR_ERR_INVALID_ARGUMENT, "too many staged rows");
-    quantize_vstep_kernel<<<(unsigned)blocks, 256, 0, s>>>(st32, L.pitch_u, L.pitch_v,
+    quantize_vstep_kernel<<<(unsigned)blocks, 256, 0, s>>>(st32, nu, nv, L.pitch_u, L.pitch_v,
                                                            view_scale, (unsigned *)staged,
                                                            dtype == CTR_BF16);
     CTR_CHECK_LAUNCH();
@@ -1614,8 +1627,12 @@ int ctr_filter_projections(void *stream, ctr_workspace *ws, const float *proj, i
     }
     const long long rows = (long long)L.pitch_v * n_views;
     CTR_REQUIRE(rows < (1LL << 31), CTR_ERR_INVALID_ARGUMENT, "too many staged rows");
-    zero_border_kernel<<<(unsigned)rows, 64, 0, s>>>(st32, nu, nv, L.pitch_u, L.pitch_v);
-    CTR_CHECK_LAUNCH();
+    if (staged_dtype == CTR_F32) {
+        // the caller's buffer: its border must read zero (the filter writes
+        // the detector texels only); 16-bit quantisation reads no border
+        zero_border_kernel<<<(unsigned)rows, 64, 0, s>>>(st32, nu, nv, L.pitch_u, L.pitch_v);
+        CTR_CHECK_LAUNCH();
+    }
     // 16-bit storage: the filter records each row's max |q|, reduced per
     // view into view_scale before quantisation
     const bool fused_max = staged_dtype != CTR_F32;

@@ -23,6 +23,8 @@ Checks:
   coordinate lies within rounding of an integer can land in the other cell
   than the FP64 oracle puts it in: the max-rel bound is the FP32 bar 1e-4
   for FP16 and 4e-4 for BF16, whose steps carry 8 bits;
+* the staging ignores what the workspace's FP32 buffer or the caller's FP32
+  staged buffer held before (borders included);
 * packed launches equal single launches bit for bit;
 * per-workspace options hold per thread: two threads with two workspaces and
   two kernel choices each reproduce their own result.
@@ -226,6 +228,38 @@ def test_16bit_forms_near_fp32_on_smooth_data(ctr, orc, dev):
         e = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
         print(f"[storage16] ball {name}: rel RMSE {e:.2e}", flush=True)
         assert e < 1e-3, (name, e)
+
+
+@pytest.mark.parametrize("dt", [0, 1, 2])
+def test_filter_staging_ignores_old_buffer_contents(ctr, dev, dt):
+    """The filter writes the detector texels of its FP32 staging buffer only.
+    A workspace whose staging buffer last held a larger scan (non-zero where
+    this scan's border lies), or an FP32 staged buffer full of garbage, gives
+    the same staged texels and scales as fresh buffers, borders included."""
+    import torch
+    big = _cone(ctr, nu=100, nv=70, n_views=40)
+    g = _cone(ctr, nu=70, nv=37, n_views=30)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(11 + dt)
+    pb = torch.rand((big.n_views, big.nv, big.nu), generator=gen, device=dev) + 1.0
+    p = torch.randn((g.n_views, g.nv, g.nu), generator=gen, device=dev)
+    used = ctr.Workspace()
+    ctr.filter_projections(pb, big, ctr.RAM_LAK, 256, dt, ws=used)
+    _, _, nbytes = ctr.staged_layout(g.nu, g.nv, g.n_views, dt)
+    dirty = torch.full((nbytes // 4,), 7.0, device=dev).view(torch.uint8)
+    got, vs = ctr.filter_projections(p, g, ctr.RAM_LAK, 256, dt, ws=used, staged=dirty)
+    ref, vs_ref = ctr.filter_projections(p, g, ctr.RAM_LAK, 256, dt, ws=ctr.Workspace())
+    torch.cuda.synchronize()
+    assert torch.equal(got, ref)
+    if dt:
+        assert torch.equal(vs, vs_ref)
+    w = got.view(torch.int32).view(g.n_views, *ctr.staged_layout(g.nu, g.nv, g.n_views, dt)[1::-1])
+    w = w.cpu().numpy()
+    if dt == 0:
+        assert (w[:, 0] == 0).all() and (w[:, g.nv + 1:] == 0).all()
+    else:  # the first row holds (0, first data row): its value half is zero
+        assert (w[:, 0] & 0xFFFF == 0).all() and (w[:, g.nv + 1:] == 0).all()
+    assert (w[:, :, 0] == 0).all() and (w[:, :, g.nu + 1:] == 0).all()
 
 
 def test_packed_16bit_equals_single(ctr, dev):
