@@ -1066,18 +1066,25 @@ static int launch_filter_pairs(hipStream_t s, const FilterJob &J, int log2n)
 // Zeroes the border texels of an fp32 staged buffer: rows 0, nv+1, nv+2 of
 // every view entirely, columns 0 and nu+1 .. pitch_u-1 of the others.  One
 // 64-thread workgroup per staged row (pitch_u - nu <= 35 border columns).
-__global__ __launch_bounds__(64) void zero_border_kernel(float *st, int nu, int nv, int pitch_u,
-                                                         int pitch_v)
+// Zeroes the border of an fp32 staged buffer: 8 staged rows per 256-thread
+// workgroup, 32 threads per row.  A border row is zeroed whole; a detector
+// row gets column 0 (thread 0) and columns nu + 1 .. pitch_u - 1 (threads
+// 1..31, 31 columns apart).
+constexpr int kBorderRowsPerBlock = 8;
+__global__ __launch_bounds__(256) void zero_border_kernel(float *st, long long rows, int nu,
+                                                          int nv, int pitch_u, int pitch_v)
 {
-    const unsigned row = blockIdx.x;  // staged row over all views
-    const int r = (int)(row % (unsigned)pitch_v);
+    const long long row = (long long)blockIdx.x * kBorderRowsPerBlock + (threadIdx.x >> 5);
+    if (row >= rows) return;
+    const int j = (int)(threadIdx.x & 31);
+    const int r = (int)(row % pitch_v);
     float *p = st + (size_t)row * pitch_u;
     if (r == 0 || r > nv) {
-        for (int c = threadIdx.x; c < pitch_u; c += 64) p[c] = 0.0f;
-    } else if (threadIdx.x == 0) {
+        for (int c = j; c < pitch_u; c += 32) p[c] = 0.0f;
+    } else if (j == 0) {
         p[0] = 0.0f;
-    } else if (nu + (int)threadIdx.x < pitch_u) {
-        p[nu + threadIdx.x] = 0.0f;
+    } else {
+        for (int c = nu + j; c < pitch_u; c += 31) p[c] = 0.0f;
     }
 }
 
@@ -1630,7 +1637,8 @@ int ctr_filter_projections(void *stream, ctr_workspace *ws, const float *proj, i
     if (staged_dtype == CTR_F32) {
         // the caller's buffer: its border must read zero (the filter writes
         // the detector texels only); 16-bit quantisation reads no border
-        zero_border_kernel<<<(unsigned)rows, 64, 0, s>>>(st32, nu, nv, L.pitch_u, L.pitch_v);
+        zero_border_kernel<<<grid1d(rows, kBorderRowsPerBlock), 256, 0, s>>>(
+            st32, rows, nu, nv, L.pitch_u, L.pitch_v);
         CTR_CHECK_LAUNCH();
     }
     // 16-bit storage: the filter records each row's max |q|, reduced per
