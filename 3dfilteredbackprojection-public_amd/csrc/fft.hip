@@ -1189,8 +1189,8 @@ constexpr int kQuantizeBand = 16;  // staged rows per workgroup of quantize_vste
 
 // One workgroup per band of kQuantizeBand staged rows of one view (blockIdx.x
 // = view * bands + band), four texels per thread: every row is read once
-// and kept in registers as the previous row's "next" (the band reads one
-// row past its end).  Only the detector texels (rows 1..nv, columns 1..nu)
+// into registers, with the row below the band (rows past the view read
+// zero).  Only the detector texels (rows 1..nv, columns 1..nu)
 // of `st` are read; the border is taken as zero, whatever `st` holds there.
 __global__ __launch_bounds__(256) void quantize_vstep_kernel(const float *st, int nu, int nv,
                                                              int pitch_u, int pitch_v,
@@ -1221,16 +1221,21 @@ __global__ __launch_bounds__(256) void quantize_vstep_kernel(const float *st, in
             return make_float4(m0 ? v.x : 0.0f, m1 ? v.y : 0.0f, m2 ? v.z : 0.0f,
                                m3 ? v.w : 0.0f);
         };
-        float4 a = row(r0);
-        for (int r = r0; r < r1; ++r) {
-            const float4 b = row(r + 1);
+        // all the band's rows (and the one below it) are loaded before any
+        // is used, so that their loads are in flight together
+        float4 q[kQuantizeBand + 1];
+#pragma unroll
+        for (int i = 0; i <= kQuantizeBand; ++i) q[i] = row(r0 + i);
+#pragma unroll
+        for (int i = 0; i < kQuantizeBand; ++i) {
+            if (r0 + i >= r1) break;
+            const float4 a = q[i], b = q[i + 1];
             uint4 o;
             o.x = pack_vstep(a.x * inv, b.x * inv, bf16);
             o.y = pack_vstep(a.y * inv, b.y * inv, bf16);
             o.z = pack_vstep(a.z * inv, b.z * inv, bf16);
             o.w = pack_vstep(a.w * inv, b.w * inv, bf16);
-            *reinterpret_cast<uint4 *>(dst + (size_t)r * pitch_u + c) = o;
-            a = b;
+            *reinterpret_cast<uint4 *>(dst + (size_t)(r0 + i) * pitch_u + c) = o;
         }
     }
 }
