@@ -340,6 +340,19 @@ def test_lds_tile_order_covers_every_tile(ctr, dev):
     assert np.isfinite(a).all()
     assert np.abs(b).max() > 0
     assert rel_max(a, b) < 5e-6
+    # a packed launch over the same job and a small one (1573 + 18 tiles)
+    # gives each job exactly its single launch
+    g2 = _cone(ctr, X=40, Y=64, Z=36, rotation_offset=0.3)
+    q2 = torch.randn((g2.n_views, g2.nv, g2.nu), generator=gen, device=dev)
+    staged2, _ = ctr.stage_projections(q2, g2, ctr.F32)
+    v1 = torch.full((g.Z, g.Y, g.X), float("nan"), device=dev)
+    v2 = torch.full((g2.Z, g2.Y, g2.X), float("nan"), device=dev)
+    ctr.backproject_packed([(staged, g, None, 0.01, v1, False),
+                            (staged2, g2, None, 0.02, v2, True)], ctr.F32)
+    s2 = ctr.backproject(staged2, g2, ctr.F32, scale=0.02, clip=True)
+    torch.cuda.synchronize()
+    assert np.array_equal(v1.cpu().numpy(), a)
+    assert torch.equal(v2, s2)
 
 
 @pytest.mark.parametrize("X,Y,Z,n_views", [(37, 45, 29, 130), (33, 64, 18, 1)])
