@@ -55,6 +55,19 @@ HBM_PEAK_GBPS = 8000.0
 VALU_PEAK_GINSTR = 157.3e3 / 2 / 64
 WORKLOAD = "fdk_512"
 FFT_LENGTH = 2048
+# the larger FDK configurations `--workload` can time in place of the
+# headline one: (FFT length, ellipsoids of the phantom, semi-axes in voxels,
+# description); the headline line is always the 512^3 one
+WORKLOADS = {
+    "fdk_512": (2048, 30, (5, 120), "FDK cone 512x512x512 @0.5mm, 720 views/360deg, "
+                                    "768x768 panel @0.8mm, SOD 500, SDD 1000, FFT 2048, Ram-Lak"),
+    "fdk_1024_s": (2048, 60, (10, 240), "FDK cone 1024x1024x1024 @0.25mm, 1200 views/360deg, "
+                                        "1024x1024 panel @0.6mm, SOD 500, SDD 1000, FFT 2048, "
+                                        "Ram-Lak"),
+    "fdk_1440": (4096, 60, (15, 340), "FDK cone 1440x1470x1440 @0.25mm, 1440 views/360deg, "
+                                      "1536x1536 panel @0.4mm, SOD 500, SDD 1000, FFT 4096, "
+                                      "Ram-Lak"),
+}
 PROFILES = os.path.join(ROOT, "profiles")
 
 
@@ -99,7 +112,8 @@ def make_inputs(ctr, g, dev, seed=20240602):
     """Phantom -> forward projection (timed: the forward-projector line) ->
     Poisson noise and log."""
     import torch
-    ell = ctr.random_ellipsoids(seed, 30, (g.X, g.Y, g.Z), g.voxel, (5, 120))
+    _, n_ell, semi, _ = WORKLOADS[WORKLOAD]
+    ell = ctr.random_ellipsoids(seed, n_ell, (g.X, g.Y, g.Z), g.voxel, semi)
     ell[:, 9] *= 0.02  # density 1.0 = 0.02 / mm
     vol = torch.zeros((g.Z, g.Y, g.X), dtype=torch.float32, device=dev)
     ctr.render_ellipsoids(ell, vol, g.volume_width)
@@ -234,6 +248,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--mode", choices=["fdk", "serving", "copy-probe"], default="fdk")
+    ap.add_argument("--workload", choices=list(WORKLOADS), default="fdk_512",
+                    help="FDK configuration to time (the headline is fdk_512; the others "
+                         "print the same line without the CPU baseline, the serving stream "
+                         "and the counter-based figures, which are the 512^3 scan's)")
     ap.add_argument("--requests", type=int, default=1000)
     ap.add_argument("--rate", type=float, default=4.0)
     ap.add_argument("--pack", type=int, default=4)
@@ -273,6 +291,12 @@ def main():
                                      args.max_slots, args.chunks)), flush=True)
         return
 
+    global WORKLOAD, FFT_LENGTH
+    WORKLOAD = args.workload
+    FFT_LENGTH = WORKLOADS[WORKLOAD][0]
+    headline = WORKLOAD == "fdk_512"
+    if not headline:
+        args.no_cpu_baseline = args.no_serving = True
     dev = torch.device("cuda:0")
     g = ctr.config_geometry(WORKLOAD)
     scale = g.fbp_scale(FFT_LENGTH)
@@ -313,7 +337,8 @@ def main():
         h2d.append(e0.elapsed_time(e1))
     del host
     h2d_ms = statistics.median(h2d)
-    h2d_gbps = proj.numel() * 4 / (h2d_ms * 1e-3) / 1e9
+    proj_bytes = proj.numel() * 4
+    h2d_gbps = proj_bytes / (h2d_ms * 1e-3) / 1e9
 
     ms = statistics.mean(f16_ms)
     gups = g.updates / (ms * 1e-3) / 1e9
@@ -322,6 +347,8 @@ def main():
     bp_ms32 = statistics.mean(f32_bp)
 
     def valu(form, t_ms):
+        if not headline:
+            return None, None
         per_update = profile_value("backproject_fdk512_sq_counters.md",
                                    f"{form}_valu_per_update")
         if per_update is None:
@@ -339,15 +366,17 @@ def main():
         mix_ceiling = profile_value("backproject_fdk512_sq_counters.md",
                                     "fp16_mix_issue_ceiling_per_ns")
         mix_source = "profiles/backproject_fdk512_sq_counters.md"
-    traffic16 = profile_value("backproject_fdk512_traffic.md", "fp16_bytes_per_launch")
-    traffic32 = profile_value("backproject_fdk512_traffic.md", "fp32_bytes_per_launch")
+    traffic16 = traffic32 = fwd_beyond_l2 = None
+    if headline:
+        traffic16 = profile_value("backproject_fdk512_traffic.md", "fp16_bytes_per_launch")
+        traffic32 = profile_value("backproject_fdk512_traffic.md", "fp32_bytes_per_launch")
+        fwd_beyond_l2 = profile_value("forward_fdk512_counters.md", "slab_beyond_l2_bytes")
     tap_bytes16 = (4 * 2 + 8.0 / g.n_views) * g.updates
     compulsory = ctr.staged_layout(g.nu, g.nv, g.n_views)[2] + 4.0 * g.X * g.Y * g.Z
 
     # forward projector on the same scan: rays and trilinear samples per
     # second; bytes from beyond L2 against reading the volume once per view
     rays = g.nu * g.nv * g.n_views
-    fwd_beyond_l2 = profile_value("forward_fdk512_counters.md", "slab_beyond_l2_bytes")
     forward = {"kernel": "forward_slab_kernel (step 1 voxel)", "ms": round(fwd_ms, 2),
                "grays_per_s": round(rays / (fwd_ms * 1e-3) / 1e9, 3),
                "gsamples_per_s": round(rays * g.X / (fwd_ms * 1e-3) / 1e9, 1),
@@ -381,8 +410,11 @@ def main():
             f"p99 {serving_line['latency_ms']['p99']} ms")
 
     out = {
-        "metric": "FDK GUPS 512^3 x 720 views (FP16-stored filtered projections, FP32 "
-                  "interpolation and accumulation)",
+        "metric": (f"FDK GUPS {g.X}x{g.Y}x{g.Z} x {g.n_views} views (FP16-stored filtered "
+                   "projections, FP32 interpolation and accumulation)"
+                   if not headline else
+                   "FDK GUPS 512^3 x 720 views (FP16-stored filtered projections, FP32 "
+                   "interpolation and accumulation)"),
         "value": round(gups, 3),
         "unit": "GUPS",
         "n_gpus": 1,
@@ -393,9 +425,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: 30 seeded ellipsoids, ray-driven projections, Poisson I0=1e5",
-        "config": {"workload": "FDK cone 512x512x512 @0.5mm, 720 views/360deg, "
-                               "768x768 panel @0.8mm, SOD 500, SDD 1000, FFT 2048, Ram-Lak",
+        "data": f"synthetic: {WORKLOADS[WORKLOAD][1]} seeded ellipsoids, ray-driven "
+                "projections, Poisson I0=1e5",
+        "config": {"workload": WORKLOADS[WORKLOAD][3],
                    "storage": "fp16 value-and-step texels + per-view scale"},
         "latency_ms": {"p50": round(statistics.median(f16_ms), 3),
                        "p99": round(float(np.percentile(f16_ms, 99)), 3)},
@@ -415,8 +447,8 @@ def main():
         "pcie_inclusive": {"h2d_ms": round(h2d_ms, 3),
                            "h2d_gb_per_s": round(h2d_gbps, 1),
                            "gups": round(g.updates / ((ms + h2d_ms) * 1e-3) / 1e9, 3),
-                           "note": "pinned host -> HBM copy of the 1.70 GB projections plus "
-                                   "the FP16 step"},
+                           "note": f"pinned host -> HBM copy of the "
+                                   f"{proj_bytes / 1e9:.2f} GB projections plus the FP16 step"},
         "roofline": {
             "bound": "valu",
             "achieved": round(achieved16, 1) if achieved16 else None,
