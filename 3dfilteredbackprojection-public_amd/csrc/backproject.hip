@@ -662,7 +662,9 @@ __device__ inline void lds_run_accumulate(float (&acc)[TY], const LdsRun &R, con
         auto issue = [&](int set, int t0) {
 #pragma unroll
             for (int q = 0; q < kB; ++q) {
-                const float x = fmaf(R.dv, (float)(t0 + q), R.f);
+                // the first voxel's row is f itself (the FMA with 0 is not
+                // folded away by the compiler)
+                const float x = t0 + q == 0 ? R.f : fmaf(R.dv, (float)(t0 + q), R.f);
                 tw[set][q] = __builtin_amdgcn_fractf(x);
                 lds_taps_f32(lds, __umul24((unsigned)x, lp4) + R.base, ta[set][q], tb[set][q]);
             }
@@ -694,7 +696,9 @@ __device__ inline void lds_run_accumulate(float (&acc)[TY], const LdsRun &R, con
         auto issue = [&](int set, int t0) {
 #pragma unroll
             for (int q = 0; q < kB; ++q) {
-                const float x = fmaf(R.dv, (float)(t0 + q), R.f);
+                // the first voxel's row is f itself (the FMA with 0 is not
+                // folded away by the compiler)
+                const float x = t0 + q == 0 ? R.f : fmaf(R.dv, (float)(t0 + q), R.f);
                 tw[set][q] = __builtin_amdgcn_fractf(x);
                 const unsigned *P = reinterpret_cast<const unsigned *>(
                     reinterpret_cast<const char *>(lds) + __umul24((unsigned)x, lp4) + R.base);
