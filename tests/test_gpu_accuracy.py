@@ -18,7 +18,7 @@ arrays between the stages (the same interchange format the product uses).
   the phantom (the centre, and one 195 voxels off the axis near the top of
   the volume, at a high cone angle).
 The 1440^3 workload's three forms are checked in
-test_gpu_fullsize.test_c3_storage_forms_vs_oracle.
+test_gpu_fullsize.test_1440_storage_forms_vs_oracle.
 
 Each case prints one '[accuracy]' line per form.  Gates: FP32 max-rel
 <= 1e-4; 16-bit forms rel RMSE <= 1e-3.

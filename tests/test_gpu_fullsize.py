@@ -81,7 +81,7 @@ def test_full_size_fp16_storage_crops(ctr, orc, dev):
     assert np.abs(_crop(vol, box) - ref).max() < 1e-3 * peak
 
 
-def test_c3_storage_forms_vs_oracle(ctr, orc, dev):
+def test_1440_storage_forms_vs_oracle(ctr, orc, dev):
     """The 1440 x 1470 x 1440 scan (1440 views of 1536^2, FFT 4096): a body
     ellipsoid (semi-axes 140 / 170 / 140 mm, density 0.02 / mm) with 60 random
     ellipsoids inside the volume, forward projected, FDK-weighted and
@@ -105,7 +105,7 @@ def test_c3_storage_forms_vs_oracle(ctr, orc, dev):
     proj = ctr.forward_project(phantom, g, step=2.0)
     del phantom
     torch.cuda.synchronize()
-    print("[c3] projections ready", flush=True)
+    print("[1440^3] projections ready", flush=True)
     box = ((688, 752), (703, 767), (688, 752))
     scale = g.fbp_scale(4096)
     forms = (("fp32", ctr.F32), ("fp16", ctr.F16), ("bf16", ctr.BF16))

@@ -157,15 +157,15 @@ def test_serving_at_real_sizes(ctr, orc, dev):
     srv = serving.Server(budget_bytes=200e9, pack=3, keep_results=True,
                          max_slots={"small": 2, "medium": 1, "large": 1})
     srv.warm(["small", "medium", "large"])
-    print("[c4] slots warm", flush=True)
+    print("[serving sizes] slots warm", flush=True)
     srv.run(mix)
-    print("[c4] mix served", flush=True)
+    print("[serving sizes] mix served", flush=True)
     for r in mix:
         alone = serving.Request(100 + r.rid, r.cls, r.seed, 0.0)
         srv.run([alone])
         assert torch.equal(alone.result, r.result), (r.cls, r.seed)
         assert alone.checksum == r.checksum
-    print("[c4] alone == mixed", flush=True)
+    print("[serving sizes] alone == mixed", flush=True)
     large = mix[0]
     g = srv.geoms["large"]
     assert (g.X, g.n_views, g.nu) == (1024, 1200, 1024)
@@ -188,6 +188,6 @@ def test_serving_at_real_sizes(ctr, orc, dev):
     (i0, i1), (j0, j1), (k0, k1) = box
     got = large.result[k0:k1, j0:j1, i0:i1].cpu().numpy()
     m = error_metrics(got, ref)
-    print(f"[c4] 1024^3 crop vs oracle (FP16 store): {format_metrics(m)}", flush=True)
+    print(f"[serving sizes] 1024^3 crop vs oracle (FP16 store): {format_metrics(m)}", flush=True)
     assert np.abs(ref).max() > 0
     assert m["rel_rmse"] < 1e-3
