@@ -249,16 +249,13 @@ __device__ inline void note_row_max(const FilterJob &J, float ma, float mb, int 
 
 // view_scale[n] = max over the rows of view n of row_max (1 when 0).
 __global__ __launch_bounds__(256) void view_max_from_rows_kernel(const float *row_max, int nv,
-                                                                 int n_views, int layout,
                                                                  float *view_scale)
 {
+    // row maxima in projection order (row = view * nv + iv), as the filter
+    // numbers its rows in either input layout
     const int n = blockIdx.x;
     float m = 0.0f;
-    for (int iv = threadIdx.x; iv < nv; iv += blockDim.x) {
-        const size_t r = layout == CTR_LAYOUT_PROJECTIONS ? (size_t)n * nv + iv
-                                                          : (size_t)iv * n_views + n;
-        m = fmaxf(m, row_max[r]);
-    }
+    for (int iv = threadIdx.x; iv < nv; iv += blockDim.x) m = fmaxf(m, row_max[(size_t)n * nv + iv]);
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
     __shared__ float part[4];
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
@@ -282,15 +279,21 @@ __device__ inline float fdk_weight_at(double sdd, double vv, double cu, double d
     return (float)(sdd * y);
 }
 
+// Rows are numbered in projection order, r = view * nv + iv, whatever the
+// input layout, and transformed in pairs (2p, 2p + 1): the same two rows
+// share a complex FFT for projections and for sinograms, so both layouts
+// give the same staged values bit for bit.  row_coords gives (view, iv) of
+// row r, input_row its row in the input.
 __device__ inline void row_coords(const FilterJob &J, int r, int &view, int &iv)
 {
-    if (J.layout == CTR_LAYOUT_PROJECTIONS) {
-        iv = r % J.nv;
-        view = r / J.nv;
-    } else {
-        view = r % J.n_views;
-        iv = r / J.n_views;
-    }
+    iv = r % J.nv;
+    view = r / J.nv;
+}
+
+__device__ inline size_t input_row(const FilterJob &J, int r)
+{
+    if (J.layout == CTR_LAYOUT_PROJECTIONS) return (size_t)r;
+    return (size_t)(r % J.nv) * J.n_views + r / J.nv;
 }
 
 template <int LOG2N>
@@ -313,8 +316,8 @@ __global__ __launch_bounds__(kFftThreads) void filter_pairs_kernel(FilterJob J)
         vv0 = J.sdd2 + v0 * v0;
         vv1 = J.sdd2 + v1 * v1;
     }
-    const float *a_row = J.in + (size_t)r0 * J.width;
-    const float *b_row = J.in + (size_t)r1 * J.width;
+    const float *a_row = J.in + input_row(J, r0) * J.width;
+    const float *b_row = J.in + input_row(J, has1 ? r1 : r0) * J.width;
     const double sdd = J.fdk ? sqrt(J.sdd2) : 0.0;
 
     load_twiddles<LOG2N>(T);
@@ -545,8 +548,8 @@ __global__ __launch_bounds__((1 << LOG2N) / 8) void filter_stockham_kernel(Filte
         vv1 = J.sdd2 + v1 * v1;
         sdd = sqrt(J.sdd2);
     }
-    const float *a_row = J.in + (size_t)r0 * J.width;
-    const float *b_row = J.in + (size_t)r1 * J.width;
+    const float *a_row = J.in + input_row(J, r0) * J.width;
+    const float *b_row = J.in + input_row(J, has1 ? r1 : r0) * J.width;
 
     // forward pass 0 (radix 8, span 1) straight from the detector rows,
     // through buffer resources sized to the rows (columns past the row and
@@ -918,11 +921,9 @@ __global__ __launch_bounds__(64 * kWaveFftWaves, CTR_WAVE_FILTER_MINW) void filt
         const int r0 = 2 * rp, r1 = r0 + 1;
         const bool has1 = r1 < J.rows;
         const int r1c = has1 ? r1 : r0;  // row coordinates of a valid row
-        const bool proj = J.layout == CTR_LAYOUT_PROJECTIONS;
-        const int iv0 = proj ? r0 % J.nv : r0 / J.n_views;
-        const int view0 = proj ? r0 / J.nv : r0 % J.n_views;
-        const int iv1 = proj ? r1c % J.nv : r1c / J.n_views;
-        const int view1 = proj ? r1c / J.nv : r1c % J.n_views;
+        int view0, iv0, view1, iv1;
+        row_coords(J, r0, view0, iv0);
+        row_coords(J, r1c, view1, iv1);
         double vv0 = 0.0, vv1 = 0.0;
         if (J.fdk) {
             const double v0 = (iv0 + 0.5 - 0.5 * J.nv) * J.delta;
@@ -934,9 +935,10 @@ __global__ __launch_bounds__(64 * kWaveFftWaves, CTR_WAVE_FILTER_MINW) void filt
         // the row (and the missing second row of an odd count) read zero,
         // so all loads are issued together without branches
         const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
-            const_cast<float *>(J.in + (size_t)r0 * J.width), (short)0, J.width * 4, 0x00020000);
+            const_cast<float *>(J.in + input_row(J, r0) * J.width), (short)0, J.width * 4,
+            0x00020000);
         const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
-            const_cast<float *>(J.in + (size_t)(has1 ? r1 : r0) * J.width), (short)0,
+            const_cast<float *>(J.in + input_row(J, r1c) * J.width), (short)0,
             has1 ? J.width * 4 : 0, 0x00020000);
         float2 v[M];
 #pragma unroll
@@ -1677,8 +1679,7 @@ int ctr_filter_projections(void *stream, ctr_workspace *ws, const float *proj, i
     J.row_max = row_max;
     if ((rc = launch_filter_pairs(s, J, log2_exact(n)))) return rc;
     if (fused_max) {
-        view_max_from_rows_kernel<<<n_views, 256, 0, s>>>(row_max, nv, n_views, layout,
-                                                           view_scale);
+        view_max_from_rows_kernel<<<n_views, 256, 0, s>>>(row_max, nv, view_scale);
         CTR_CHECK_LAUNCH();
     }
     return finish_staged(s, st32, nu, nv, n_views, staged, staged_dtype, view_scale, fused_max);

@@ -93,16 +93,16 @@ def test_filter_projections_16bit_equals_staged_fp32(ctr, dev, dt):
     """16-bit staging by the filter (per-row maxima fused into the filter,
     then the per-view scale and the value-and-step texels) equals staging
     the filter's FP32 output afterwards (ctr_stage_projections), bit for
-    bit, in both input layouts.  (The filter transforms two rows in one
-    complex FFT: two rows of a view for projections, the same row of two
-    views for sinograms, so the two layouts' FP32 outputs agree to FP32
-    rounding, not bit for bit; each is compared with its own.)"""
+    bit; and both input layouts give the same staged buffers bit for bit
+    (the filter pairs rows in projection order in either layout), with an
+    all-zero view staying exactly zero (scale 1)."""
     import torch
     g = _cone(ctr, n_views=37)
     rng = np.random.default_rng(12 + dt)
     proj = rng.standard_normal((g.n_views, g.nv, g.nu)).astype(np.float32)
     proj[5] = 0.0  # an all-zero view
     pu, pv, _ = ctr.staged_layout(g.nu, g.nv, g.n_views)
+    out = []
     for layout in (0, 1):
         src = _t(dev, proj if layout == 0 else np.ascontiguousarray(proj.transpose(1, 0, 2)))
         s32, _ = ctr.filter_projections(src, g, 0, 256, ctr.F32, fdk=True, layout=layout)
@@ -112,8 +112,10 @@ def test_filter_projections_16bit_equals_staged_fp32(ctr, dev, dt):
         torch.cuda.synchronize()
         assert torch.equal(vs, want_vs), layout
         assert torch.equal(got, want), layout
-        if layout == 0:  # the view's own rows only: exactly zero, scale 1
-            assert float(vs[5]) == 1.0
+        assert float(vs[5]) == 1.0 and not q[5].any(), layout
+        out.append((s32, got, vs))
+    assert torch.equal(out[0][0], out[1][0])
+    assert torch.equal(out[0][1], out[1][1]) and torch.equal(out[0][2], out[1][2])
 
 
 def _fdk_ref(orc, g, q, scale):
