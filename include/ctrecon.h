@@ -194,7 +194,9 @@ int ctr_filter_rows(void *stream, const float *in, float *out, int width,
  * (FBP3D.cpp:369-387).  For CTR_F16 / CTR_BF16 every view is scaled by its
  * max |value| (stored in view_scale[N], device memory; 1 for a view that is
  * all zero) before rounding; for CTR_F32 view_scale may be NULL.  `layout`
- * gives the input layout. */
+ * gives the input layout; both layouts give the same staged buffer bit for
+ * bit (rows are paired for the shared complex FFTs in projection order,
+ * view by view, in either layout). */
 int ctr_filter_projections(void *stream, ctr_workspace *ws, const float *proj,
                            int layout, int nu, int nv, int n_views,
                            int fft_length, int filter, int fdk_weight,
