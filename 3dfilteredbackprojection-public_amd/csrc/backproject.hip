@@ -485,9 +485,12 @@ __global__ __launch_bounds__(256) void backproject_kernel(BpJob J)
 #ifndef CTR_F32_BATCH
 #define CTR_F32_BATCH 2
 #endif
-// The same for the 16-bit value-and-step texels.
+// The same for the 16-bit value-and-step texels: batch 2 against batch 1,
+// kernel ms at 512^3 / 1024^3 / 1440^3 (same box): fp16 22.20 / 269.5 /
+// 801.2 against 22.32-22.48 / 272.0-272.5 / 805.5-810.1, bf16 23.86 / 286.6
+// / 835.9 against 23.93-24.07 / 287.6-289.5 / 840.0-841.0.
 #ifndef CTR_16_BATCH
-#define CTR_16_BATCH 1
+#define CTR_16_BATCH 2
 #endif
 constexpr int kRectChunk = 128; // views per refill of the rectangle table
 
