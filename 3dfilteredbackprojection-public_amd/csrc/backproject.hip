@@ -864,6 +864,11 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
 //   1024^3 x 1200: 287.6 / 362.1, 274.4-279.7 / 349.4-354.1, 275.7-276.2 /
 //     350.1-355.0.
 //   1440^3 x 1440: bands of 4 in ranges 965 / 1159; chunks of 128 828 / 1014.
+//   With chunks dealt round-robin, against bands of 4 in chunks of 128
+//   (512^3 / 1024^3 FP16 kernel, alternating runs on one box): bands of 2
+//   +1.5% / +0.8%, of 8 +0.7% / +2.1%, of 16 +2.8% / +2.2%; bands of 8 in
+//   chunks of 64 +1.5% / +1.9%, of 256 +0.4% / +1.1% (FP32 +1.7-4.6% at
+//   512^3).
 #ifndef CTR_TILE_BAND
 #define CTR_TILE_BAND 4
 #endif
