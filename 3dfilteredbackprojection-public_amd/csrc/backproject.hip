@@ -629,13 +629,59 @@ __device__ inline void stage_rect(const char *view, unsigned view_bytes, int pit
     }
 }
 
+// Row coordinates of the 16-bit LDS runs carry a bias of 128: a row x in
+// [0, 128) is held as x + 128, a float in [128, 256), whose bits are
+// 0x43000000 + floor(x) * 2^16 + the fraction in 16 bits.  The high half of
+// the bits is then 0x4300 + floor(x), and one v_mad_u32_u16 on it (op_sel
+// selects the high half) gives the LDS address, where an unbiased row needs
+// a conversion and a 24-bit multiply-add; v_fract_f32 of the biased row is
+// the row weight as before.  The row is then rounded to 2^-16 (1.5e-5
+// rows) in place of 2^-17 .. 2^-18 for rows 32 .. 127, an error far below
+// the 16-bit texels' own; FP32 runs stay unbiased and keep the finer
+// rounding.  Measured (kernel ms, 512^3 / 1024^3, alternating runs on one
+// box): FP16 21.34-21.39 / 255.1-256.0 against 22.18-22.24 / 270.0-270.1,
+// BF16 23.26-23.27 / 279.4-280.4 against 23.74-24.07 / 287.4.
+#ifndef CTR_ROW_BIAS
+#define CTR_ROW_BIAS 1
+#endif
+template <int DT>
+__device__ constexpr bool row_biased()
+{
+    return CTR_ROW_BIAS && DT != CTR_F32;
+}
+
+// LDS byte address of row floor(x) of a run: `base` is the run's address of
+// row 0 as lds_base gives it.
+template <int DT>
+__device__ inline unsigned lds_row_addr(float x, unsigned lp4, unsigned base)
+{
+    if constexpr (row_biased<DT>()) {
+        unsigned a;
+        asm("v_mad_u32_u16 %0, %1, %2, %3 op_sel:[1,0,0,0]"
+            : "=v"(a)
+            : "v"(__float_as_uint(x)), "v"(lp4), "v"(base));
+        return a;
+    } else {
+        return __umul24((unsigned)x, lp4) + base;
+    }
+}
+
+// The run's row-0 address as lds_row_addr takes it (a biased row's high
+// half counts 0x4300 rows too many; the address wraps modulo 2^32).
+template <int DT>
+__device__ inline unsigned lds_base(unsigned base, unsigned lp4)
+{
+    return row_biased<DT>() ? base - 0x4300u * lp4 : base;
+}
+
 // Per-view constants of a column run read from LDS.
 struct LdsRun {
     float cL, cR;   // (1 - wu) w2, wu w2
-    float f, dv;    // row of the first voxel relative to the rectangle, row step
+    float f, dv;    // row of the first voxel relative to the rectangle (biased for 16-bit), row step
     unsigned base;  // LDS byte address of (row 0, column of the left tap)
 };
 
+template <int DT>
 __device__ inline LdsRun lds_run(const ColumnView &c, int u0, int v0, unsigned buf_byte, int cmax)
 {
     LdsRun R;
@@ -644,7 +690,7 @@ __device__ inline LdsRun lds_run(const ColumnView &c, int u0, int v0, unsigned b
     const float wu = (float)(su - sfl);
     R.cR = c.w2 * wu;
     R.cL = c.w2 - R.cR;
-    R.f = (float)(c.sv0 - (double)v0);
+    R.f = (float)(c.sv0 - (double)v0 + (row_biased<DT>() ? 128.0 : 0.0));
     R.dv = c.dv;
     R.base = buf_byte + (unsigned)min(max((int)sfl, 0), cmax - 2) * 4u;
     return R;
@@ -658,6 +704,7 @@ __device__ inline void lds_run_accumulate(float (&acc)[TY], const LdsRun &R, con
     // Software-pipelined in batches of kB voxels: the taps of batch b + 1
     // are in flight while batch b is interpolated; the scheduling barriers
     // keep that order.
+    const unsigned base = lds_base<DT>(R.base, lp4);
     if constexpr (DT == CTR_F32) {
         constexpr int kB = CTR_F32_BATCH;
         float2 ta[2][kB], tb[2][kB];
@@ -669,7 +716,7 @@ __device__ inline void lds_run_accumulate(float (&acc)[TY], const LdsRun &R, con
                 // folded away by the compiler)
                 const float x = t0 + q == 0 ? R.f : fmaf(R.dv, (float)(t0 + q), R.f);
                 tw[set][q] = __builtin_amdgcn_fractf(x);
-                lds_taps_f32(lds, __umul24((unsigned)x, lp4) + R.base, ta[set][q], tb[set][q]);
+                lds_taps_f32(lds, lds_row_addr<DT>(x, lp4, base), ta[set][q], tb[set][q]);
             }
         };
         issue(0, 0);
@@ -704,7 +751,7 @@ __device__ inline void lds_run_accumulate(float (&acc)[TY], const LdsRun &R, con
                 const float x = t0 + q == 0 ? R.f : fmaf(R.dv, (float)(t0 + q), R.f);
                 tw[set][q] = __builtin_amdgcn_fractf(x);
                 const unsigned *P = reinterpret_cast<const unsigned *>(
-                    reinterpret_cast<const char *>(lds) + __umul24((unsigned)x, lp4) + R.base);
+                    reinterpret_cast<const char *>(lds) + lds_row_addr<DT>(x, lp4, base));
                 ta[set][q] = P[0];
                 tb[set][q] = P[1];
             }
@@ -741,6 +788,7 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
 {
     using L = LdsLayout<DT>;
     constexpr int HMAX = L::kRows, CMAX = L::kCols;
+    static_assert(HMAX < 127, "biased LDS rows hold rows below 128");
     constexpr int kBelow = DT == CTR_F32 ? 1 : 0;  // taps of row I + 1 read from LDS
     int4 *rects = reinterpret_cast<int4 *>(lds);
     unsigned *bufs = lds + L::kRectBytes / 4;
@@ -809,7 +857,7 @@ __device__ inline void backproject_lds_tile(const BpJob &J, int bx, int by, int 
             if (J.view_scale) vs *= as_constant(J.view_scale)[view];
             const ColumnView cv = column_view64(J.geom64 + (size_t)kGeom64 * view, i, k, j0, vs);
             const unsigned buf_byte = (unsigned)(L::kRectBytes + (n & 1) * L::kBufDwords * 4);
-            const LdsRun R = lds_run(cv, cur.x, cur.y, buf_byte, CMAX);
+            const LdsRun R = lds_run<DT>(cv, cur.x, cur.y, buf_byte, CMAX);
             lds_run_accumulate<DT, TY>(acc, R, lds, lp4);
         }
         any_direct |= rect_mode(cur) == RECT_DIRECT;
