@@ -629,25 +629,30 @@ __device__ inline void stage_rect(const char *view, unsigned view_bytes, int pit
     }
 }
 
-// Row coordinates of the 16-bit LDS runs carry a bias of 128: a row x in
+// Row coordinates of the FP16 LDS runs carry a bias of 128: a row x in
 // [0, 128) is held as x + 128, a float in [128, 256), whose bits are
 // 0x43000000 + floor(x) * 2^16 + the fraction in 16 bits.  The high half of
 // the bits is then 0x4300 + floor(x), and one v_mad_u32_u16 on it (op_sel
 // selects the high half) gives the LDS address, where an unbiased row needs
 // a conversion and a 24-bit multiply-add; v_fract_f32 of the biased row is
 // the row weight as before.  The row is then rounded to 2^-16 (1.5e-5
-// rows) in place of 2^-17 .. 2^-18 for rows 32 .. 127, an error far below
-// the 16-bit texels' own; FP32 runs stay unbiased and keep the finer
-// rounding.  Measured (kernel ms, 512^3 / 1024^3, alternating runs on one
-// box): FP16 21.34-21.39 / 255.1-256.0 against 22.18-22.24 / 270.0-270.1,
-// BF16 23.26-23.27 / 279.4-280.4 against 23.74-24.07 / 287.4.
+// rows) in place of 2^-17 .. 2^-18 for rows 32 .. 127.  FP32 and BF16 runs
+// stay unbiased: FP32 keeps the finer rounding, and a BF16 step carries 8
+// bits, so a sample that the coarser rounding moves across a row boundary
+// (where v + d of one row and the next row's v differ by the step's
+// rounding) would cost BF16 four times what it costs FP16.  Measured
+// (kernel ms, 512^3 / 1024^3, alternating runs on one box): FP16
+// 21.34-21.39 / 255.1-256.0 against 22.18-22.24 / 270.0-270.1; BF16 biased
+// 23.26-23.27 / 279.4-280.4 against 23.74-24.07 / 287.4, but 4.7e-4 max-rel
+// against the oracle of its dequantized texels on white noise (4e-4
+// unbiased).
 #ifndef CTR_ROW_BIAS
 #define CTR_ROW_BIAS 1
 #endif
 template <int DT>
 __device__ constexpr bool row_biased()
 {
-    return CTR_ROW_BIAS && DT != CTR_F32;
+    return CTR_ROW_BIAS && DT == CTR_F16;
 }
 
 // LDS byte address of row floor(x) of a run: `base` is the run's address of
@@ -677,7 +682,7 @@ __device__ inline unsigned lds_base(unsigned base, unsigned lp4)
 // Per-view constants of a column run read from LDS.
 struct LdsRun {
     float cL, cR;   // (1 - wu) w2, wu w2
-    float f, dv;    // row of the first voxel relative to the rectangle (biased for 16-bit), row step
+    float f, dv;    // row of the first voxel relative to the rectangle (biased for FP16), row step
     unsigned base;  // LDS byte address of (row 0, column of the left tap)
 };
 

@@ -73,11 +73,11 @@ PROBE(p_addf, EACH(ADDF))
 PROBE(p_addu, EACH(ADDU))
 PROBE(p_mad16, EACH(MAD16))
 
-// Address part of one voxel of the fp32 loop: row coordinate FMA, fract (row
-// weight), cvt (row), mad24 (LDS address).
+// Address part of one voxel of the fp32 and bf16 loops: row coordinate FMA,
+// fract (row weight), cvt (row), mad24 (LDS address).
 #define ADDR32 "v_fma_f32 %0, %8, %8, %0\n v_fract_f32 %1, %0\n v_cvt_u32_f32 %2, %0\n" \
                " v_mad_u32_u24 %3, %9, %9, %2\n"
-// The same for the 16-bit loops, whose rows are biased by 128: row
+// The same for the fp16 loop, whose rows are biased by 128: row
 // coordinate FMA, fract, and the LDS address from the row's high half
 // (v_mad_u32_u16 with op_sel).
 #define ADDR16 "v_fma_f32 %0, %8, %8, %0\n v_fract_f32 %1, %0\n" \
@@ -85,7 +85,7 @@ PROBE(p_mad16, EACH(MAD16))
 // fp16 value-and-step texels: two v_fma_mix_f32 lerps, two weighted FMAs
 PROBE(p_loop_fp16, ADDR16 MIX(4) MIX(5) FMAC(6) FMAC(7))
 // bf16 value-and-step texels: two shifts widen the values, two lerps, two FMAs
-PROBE(p_loop_bf16, ADDR16 SHL(4) SHL(5) FMAC(6) FMAC(7) FMAC(4) FMAC(5))
+PROBE(p_loop_bf16, ADDR32 SHL(4) SHL(5) FMAC(6) FMAC(7) FMAC(4) FMAC(5))
 // fp32 texels: two differences, two lerps, two weighted FMAs
 PROBE(p_loop_fp32, ADDR32 SUBF(4) SUBF(5) FMAC(6) FMAC(7) FMAC(4) FMAC(5))
 
@@ -113,7 +113,7 @@ int main()
         {"v_lshlrev_b32", p_shl, 8},       {"v_sub_f32", p_subf, 8},
         {"v_add_f32", p_addf, 8},          {"v_add_u32", p_addu, 8},
         {"v_mad_u32_u16", p_mad16, 8},     {"loop mix fp16", p_loop_fp16, 7},
-        {"loop mix bf16", p_loop_bf16, 9}, {"loop mix fp32", p_loop_fp32, 10}};
+        {"loop mix bf16", p_loop_bf16, 10}, {"loop mix fp32", p_loop_fp32, 10}};
     std::printf("VALU issue rate, wave64 instructions per ns per SIMD (%d CUs, 4 SIMDs each)\n",
                 cus);
     std::printf("%-16s %10s %10s %10s\n", "", "4 waves", "5 waves", "8 waves");

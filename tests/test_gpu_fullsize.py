@@ -192,12 +192,13 @@ def test_backproject_kernel_variants_agree(ctr, dev):
     A sample whose row coordinate lies within FP32 rounding of an integer
     can fall on either side, so the kernels agree to that rounding there:
     measured on white-noise views (row differences as large as the values)
-    about 2e-4 max-rel for BF16 and a quarter of it for FP16.  The LDS
-    kernel of the 16-bit forms also rounds its row coordinates to 2^-16
-    (a row held biased by 128 gives its LDS address in one instruction):
-    a uniform rounding of 4.4e-6 rows RMS, times white-noise row differences
-    of about 1.4 times the values, is an RMS difference of about 6e-6 of
-    the values, views being independent."""
+    about 2e-4 max-rel for BF16 and a quarter of it for FP16, with the
+    RMS difference at 3e-6 and below.  The LDS kernel of FP16 also rounds
+    its row coordinates to 2^-16 (a row held biased by 128 gives its LDS
+    address in one instruction): a uniform rounding of 4.4e-6 rows RMS,
+    times white-noise row differences of about 1.4 times the values, is an
+    RMS difference of about 6e-6 of the values, views being independent
+    (measured 1.1e-5 with the direct kernel's own rounding)."""
     import torch
     g = ctr.config_geometry("fdk_256_s")
     q = _random_views(ctr, g, dev, 5)
@@ -212,7 +213,7 @@ def test_backproject_kernel_variants_agree(ctr, dev):
         a, b = a.cpu().numpy(), b.cpu().numpy()
         print(f"[variants] dtype {dt}: max {rel_max(a, b):.3e} "
               f"rmse {rel_rmse(a, b):.3e}", flush=True)
-        bound = {ctr.F32: (5e-6, 5e-6), ctr.F16: (1e-4, 3e-5), ctr.BF16: (5e-4, 3e-5)}[dt]
+        bound = {ctr.F32: (5e-6, 5e-6), ctr.F16: (1e-4, 3e-5), ctr.BF16: (5e-4, 1e-5)}[dt]
         assert rel_max(a, b) < bound[0], dt
         assert rel_rmse(a, b) < bound[1], dt
 

@@ -18,10 +18,10 @@ Checks:
   beams.  Within a cell the texels are exactly linear, so FP32 arithmetic
   gives FP32 rounding: rel RMSE <= 1e-5 for the direct-tap kernel (16-bit
   interpolation arithmetic would show 1e-4 and more on these white-noise
-  views).  The LDS kernel of the 16-bit forms holds its row coordinates
-  biased by 128, rounded to 2^-16 of a row (a texture sampler's weights
-  carry 2^-8): on white noise, whose row differences are as large as the
-  values, that is a rel RMSE of about 1.1e-5, bounded at 2.5e-5.  Across a cell
+  views).  The LDS kernel of FP16 holds its row coordinates biased by
+  128, rounded to 2^-16 of a row (a texture sampler's weights carry 2^-8):
+  on white noise, whose row differences are as large as the values, that
+  is a rel RMSE of about 1.1e-5, bounded at 2.5e-5.  Across a cell
   boundary the lower tap of cell k (v + d of row k) differs from row
   k + 1's own value by the step's rounding, so a sample whose FP32 row
   coordinate lies within rounding of an integer can land in the other cell
@@ -43,7 +43,7 @@ pytestmark = pytest.mark.gpu
 # max-rel bounds against the dequantized oracle, and rel RMSE bounds per
 # kernel (module docstring)
 MAX_REL = {1: 1e-4, 2: 4e-4}
-RMSE = {"lds": 2.5e-5, "gather": 1e-5}
+RMSE = {(1, "lds"): 2.5e-5, (1, "gather"): 1e-5, (2, "lds"): 1e-5, (2, "gather"): 1e-5}
 
 
 def _cone(ctr, **kw):
@@ -177,7 +177,7 @@ def test_backprojection_vs_dequantized_oracle(ctr, orc, dev, X, Y, Z, n_views):
             e, r = rel_max(got, ref), rel_rmse(got, ref)
             print(f"[storage16] {X}x{Y}x{Z}x{n_views} dt={dt} {name}: max-rel {e:.2e} "
                   f"rel-rmse {r:.2e}", flush=True)
-            assert e < MAX_REL[dt] and r < RMSE[name], (dt, name, e, r)
+            assert e < MAX_REL[dt] and r < RMSE[dt, name], (dt, name, e, r)
 
 
 def test_parallel_backprojection_vs_dequantized_oracle(ctr, orc, dev):
