@@ -246,8 +246,11 @@ int ctr_stage_projections(void *stream, ctr_workspace *ws, const float *proj,
  * up in FP64 per column and view and stepped in FP32 along y; interpolation
  * and accumulation are FP32 in every format (16-bit texels widened
  * exactly): for CTR_F16 / CTR_BF16 the vertical taps of a column are v and
- * v + d of its value-and-step texel.  The sum over views runs in view
- * order, so results are reproducible bit for bit. */
+ * v + d of its value-and-step texel.  Row coordinates are FP32 numbers;
+ * for CTR_F16 the LDS-staged kernel holds them biased by 128, rounded to
+ * 2^-16 of a row (the bias gives the LDS address in one instruction).  The
+ * sum over views runs in view order, so results are reproducible bit for
+ * bit. */
 int ctr_backproject(void *stream, ctr_workspace *ws, int geometry,
                     const void *staged, int staged_dtype,
                     const float *view_scale, int nu, int nv, int n_views,
