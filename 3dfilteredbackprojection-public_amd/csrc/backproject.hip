@@ -466,6 +466,9 @@ __global__ __launch_bounds__(256) void backproject_kernel(BpJob J)
 #ifndef CTR_LDS_PITCH16
 #define CTR_LDS_PITCH16 51
 #endif
+// 16-bit rows: 60 (six workgroups per CU) measured FP16 1.1% faster at
+// 1024^3 x 1200 but 20% slower at 512^3 x 720 and 13% at 1440^3 x 1440, whose
+// taller footprints then take the direct-tap pass more often.
 #ifndef CTR_LDS_ROWS16
 #define CTR_LDS_ROWS16 72
 #endif
