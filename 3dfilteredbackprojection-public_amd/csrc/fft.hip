@@ -654,10 +654,10 @@ __global__ __launch_bounds__((1 << LOG2N) / 8) void filter_stockham_kernel(Filte
 // in LDS and each wave loops over row pairs.
 
 // One-wave filter for n = 512 .. 2048 (1) or the Stockham kernel (0); j
-// groups per exchange round; minimum waves per SIMD the compiler must keep
+// groups per exchange step; minimum waves per SIMD the compiler must keep
 // registers for.  Measured on 512^3 x 720 (FFT 2048, fp32 staging, ms):
 // Stockham 2.81, one-wave 1.94 (2 waves per SIMD) / 1.86 (3, chosen);
-// 256^3 x 360 (FFT 1024) 0.337 / 0.214; two j groups per round 0.266.
+// 256^3 x 360 (FFT 1024) 0.337 / 0.214; two j groups per exchange step 0.266.
 #ifndef CTR_WAVE_FILTER
 #define CTR_WAVE_FILTER 1
 #endif
