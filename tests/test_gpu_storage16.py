@@ -180,6 +180,40 @@ def test_backprojection_vs_dequantized_oracle(ctr, orc, dev, X, Y, Z, n_views):
             assert e < MAX_REL[dt] and r < RMSE[dt, name], (dt, name, e, r)
 
 
+def test_tall_footprints_vs_dequantized_oracle(ctr, orc, dev):
+    """Detector rows of 0.8 voxel at a magnification of 1.67 .. 1.92: a
+    32-voxel run spans 67 .. 77 rows, so the LDS kernel's rectangles reach
+    the top of its 72-row buffer (FP16 rows up to 71.95, biased to 199.95)
+    and the taller ones take the direct-tap pass.  Both kernels against the
+    FP64 oracle of the dequantized cells, as above."""
+    import torch
+    g = _cone(ctr, nu=112, nv=160, X=40, Y=64, Z=36, detector_width=89.6,
+              rotation_offset=0.4)
+    mag = (g.source_distance + g.detector_distance) / g.source_distance
+    run_rows = 32 * (g.volume_width / g.X) * mag / (g.detector_width / g.nu)
+    assert run_rows > 64
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(123)
+    p = torch.randn((g.n_views, g.nv, g.nu), generator=gen, device=dev)
+    kernels = {}
+    for name, k in (("lds", ctr.KERNEL_LDS), ("gather", ctr.KERNEL_GATHER)):
+        kernels[name] = ctr.Workspace()
+        kernels[name].set_option(ctr.OPTION_BACKPROJECT_KERNEL, k)
+    for dt in (ctr.F16, ctr.BF16):
+        staged, vs = ctr.stage_projections(p, g, dt)
+        top, bot = dequantized_cells(ctr, staged, vs, g, dt)
+        ref = orc.backproject_cone_cells(top, bot, g.source_distance, g.detector_distance,
+                                         g.detector_width, g.volume_width, g.rotation_count,
+                                         g.rotation_offset, 0.01, (g.X, g.Y, g.Z))
+        for name, ws in kernels.items():
+            got = ctr.backproject(staged, g, dt, view_scale=vs, scale=0.01,
+                                  ws=ws).cpu().numpy()
+            e, r = rel_max(got, ref), rel_rmse(got, ref)
+            print(f"[storage16] tall footprints dt={dt} {name}: max-rel {e:.2e} "
+                  f"rel-rmse {r:.2e}", flush=True)
+            assert e < MAX_REL[dt] and r < RMSE[dt, name], (dt, name, e, r)
+
+
 def test_parallel_backprojection_vs_dequantized_oracle(ctr, orc, dev):
     """Parallel beams (LDS kernel: 40 voxels along y; and the 2D direct-tap
     kernel on a one-row detector) against the oracle of the dequantized
